@@ -1,0 +1,165 @@
+/*
+ * libmvdecon_hip: MI355X (gfx950) implementation of the efficient-Bayesian
+ * multiview deconvolution path of fiji/SPIM_Registration.
+ *
+ * Two groups of entry points, all plain C:
+ *
+ *  1. The JNA ABI the reference already binds.  The Java interfaces
+ *     spim.process.cuda.CUDAStandardFunctions and CUDAFourierConvolution map
+ *     these symbols by name (Java int -> int32, long -> int64, float[] ->
+ *     float*, int[] -> int*, byte[] -> char*).  The library can be selected in
+ *     the reference's native-library dialog unchanged.
+ *
+ *  2. A session API that keeps a whole deconvolution resident in HBM (views,
+ *     weights, estimate, kernel spectra) and runs the MVDeconvolution
+ *     iteration on the GPU.  It mirrors the reference operator API:
+ *     MVDeconFFT (one view: image, weight, kernel, block size),
+ *     MVDeconInput.init (kernel preparation per PSF type) and
+ *     MVDeconvolution (initial estimate, iterations, Tikhonov, final mask).
+ *
+ * Arrays are x-fastest float32 volumes; host arrays are copied in and out,
+ * the library never keeps or frees caller pointers.  Session functions return
+ * 0 on success and a negative code on failure (message: mvd_last_error()).
+ * A session belongs to one device; calls on one session must not overlap.
+ */
+#ifndef MVDECON_H
+#define MVDECON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. JNA ABI -------------------------------------------------------- */
+
+/* Replaces CUDAStandardFunctions.getNumDevicesCUDA
+ * (spim/process/cuda/CUDAStandardFunctions.java:41): number of GPUs, -1 if the
+ * driver cannot be queried. */
+int getNumDevicesCUDA(void);
+
+/* Replaces CUDAStandardFunctions.getNameDeviceCUDA (CUDAStandardFunctions.java:42):
+ * writes a NUL-terminated name into the caller's 256-byte buffer. */
+void getNameDeviceCUDA(int devCUDA, char* name);
+
+/* Replaces CUDAStandardFunctions.getMemDeviceCUDA (CUDAStandardFunctions.java:43):
+ * total device memory in bytes, -1 on error. */
+long long getMemDeviceCUDA(int devCUDA);
+
+/* Replaces CUDAStandardFunctions.getFreeMemDeviceCUDA (CUDAStandardFunctions.java:44):
+ * free device memory in bytes, -1 on error. */
+long long getFreeMemDeviceCUDA(int devCUDA);
+
+/* Replace CUDAStandardFunctions.getCUDAcomputeCapability{Major,Minor}Version
+ * (CUDAStandardFunctions.java:36-37): the HIP device major/minor (9/5 on gfx950);
+ * the reference only uses them to pick a default device. */
+int getCUDAcomputeCapabilityMajorVersion(int devCUDA);
+int getCUDAcomputeCapabilityMinorVersion(int devCUDA);
+
+/* Replaces CUDAFourierConvolution.convolution3DfftCUDAInPlace
+ * (spim/process/cuda/CUDAFourierConvolution.java:31), called per block at
+ * deconv/MVDeconFFTThreads.java:86-89 and :109-112.
+ * im: block of imDim = {nz, ny, nx} floats, x-fastest; kernel: {kz, ky, kx}.
+ * Convolves im with kernel (true convolution, kernel centre at k/2 per axis,
+ * circular over the block padded to an FFT-friendly size) and writes the
+ * result into im.  Voxels at least k/2 from every block face -- the region
+ * Block.pasteBlock copies back -- equal the linear convolution exactly; the
+ * other voxels hold the circular result and are discarded by the caller.
+ * On failure im is left unchanged and the error is printed to stderr (the
+ * Java signature has no error channel). */
+void convolution3DfftCUDAInPlace(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA);
+
+/* Replaces CUDAFourierConvolution.convolution3DfftCUDA
+ * (CUDAFourierConvolution.java:30; declared, never called by the reference).
+ * Same result as the in-place call, returned in a new buffer the caller
+ * releases with mvd_free(); NULL on failure. */
+float* convolution3DfftCUDA(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA);
+void mvd_free(void* p);
+
+/* ---- 2. Session API ----------------------------------------------------- */
+
+typedef struct mvd_session mvd_session;
+
+/* MVDeconFFT.PSFTYPE (deconv/MVDeconFFT.java:49), same order. */
+enum {
+  MVD_PSF_OPTIMIZATION_II = 0,
+  MVD_PSF_OPTIMIZATION_I = 1,
+  MVD_PSF_EFFICIENT_BAYESIAN = 2,
+  MVD_PSF_INDEPENDENT = 3
+};
+
+/* How the weights given to mvd_add_view are used
+ * (ProcessForDeconvolution.WeightType, deconv/ProcessForDeconvolution.java:221-243). */
+enum {
+  MVD_WEIGHTS_NONE = 0,        /* w = 1 everywhere (NO_WEIGHTS) */
+  MVD_WEIGHTS_AS_GIVEN = 1,    /* weights used as passed (already normalised) */
+  MVD_WEIGHTS_PRECOMPUTED = 2, /* normalised in place: w_v /= sum_u w_u (WeightNormalizer.ApplyDirectly) */
+  MVD_WEIGHTS_VIRTUAL = 3      /* w_v / max(sum, 1) evaluated per voxel (NormalizingRandomAccess) */
+};
+
+/* new MVDeconInput + image geometry: nviews views of nx*ny*nz voxels on `device`. */
+int mvd_create(int device, int nx, int ny, int nz, int nviews, mvd_session** out);
+int mvd_destroy(mvd_session* s);
+
+/* new MVDeconFFT(image, weight, kernel, ...) + MVDeconInput.add
+ * (deconv/MVDeconFFT.java:79-172, deconv/MVDeconInput.java:48-54).
+ * Views are added in order; weight may be NULL (then w = 1).  Kernel sizes must
+ * be odd (cuda/BlockGeneratorFixedSizePrecise.java:43). */
+int mvd_add_view(mvd_session* s, const float* img, const float* weight, const float* psf, int kx, int ky, int kz);
+
+/* Weight handling before preparation (ProcessForDeconvolution.java:325-405):
+ * mode MVD_WEIGHTS_*; osem_index as EfficientBayesianBased.osemspeedupIndex
+ * (deconv/EfficientBayesianBased.java:1023-1042, ProcessForDeconvolution.java:353-356):
+ * 0 = 1, 1 = minimal number of overlapping views, 2 = average number,
+ * 3 = osem_value.  norm_threads is the reference's thread count (the
+ * average is the mean over its 2*norm_threads portions).
+ * min_avg_out (nullable) receives {min overlapping views, avg overlapping views}. */
+int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value, int norm_threads,
+                    double* min_avg_out);
+
+/* MVDeconInput.init(PSFTYPE) (deconv/MVDeconInput.java:62-68 -> MVDeconFFT.init,
+ * deconv/MVDeconFFT.java:183-324) plus block split.  block_xyz NULL = the whole
+ * image as one block of img+k-1 (MVDeconFFT.java:144-165); otherwise
+ * BlockGeneratorFixedSizePrecise with that block size.  norm_threads is the
+ * reference's Threads.numThreads(): AdjustInput.normImg counts the first of
+ * its 2*norm_threads portions twice (deconv/AdjustInput.java:93-97). */
+int mvd_prepare(mvd_session* s, int psf_type, double lambda, const int* block_xyz, int norm_threads);
+
+/* Initial estimate (MVDeconvolution.java:116-148): psi0 == NULL fuses the
+ * first iteration (average of views with data, 0.5 if NaN); otherwise psi0 is
+ * used as given.  avg_out (nullable) receives the average. */
+int mvd_init_psi(mvd_session* s, const float* psi0, double* avg_out);
+
+/* n calls of MVDeconvolution.runIteration (MVDeconvolution.java:349-465).
+ * stats (nullable) receives n * nviews * {sum change, max change}. */
+int mvd_iterate(mvd_session* s, int n, double* stats);
+
+/* Copy the estimate out; masked != 0 applies the final mask of
+ * MVDeconvolution.java:201-208 (0 where no view has data) to the copy. */
+int mvd_get_psi(mvd_session* s, float* out, int masked);
+
+/* MVDeconFFT.getKernel1/getKernel2 (MVDeconFFT.java:373-374) of a view. */
+int mvd_get_kernels(mvd_session* s, int view, float* kernel1, float* kernel2);
+
+/* MVDeconFFT.setKernel (MVDeconFFT.java:364-369): replace a view's PSF and
+ * re-run its init (normalisation, kernel2, spectra). */
+int mvd_set_psf(mvd_session* s, int view, const float* psf);
+
+/* Per-kernel-class GPU timing (HIP events on the session stream).
+ * mvd_set_timing(s, 1) enables and resets; mvd_get_timing fills up to cap
+ * classes: total ms and launch counts; returns the number of classes.
+ * mvd_timing_name(i) names class i. */
+int mvd_set_timing(mvd_session* s, int on);
+int mvd_get_timing(mvd_session* s, double* total_ms, long long* launches, int cap);
+const char* mvd_timing_name(int cls);
+
+/* Geometry summary after mvd_prepare for one view: number of blocks and the
+ * FFT size used; out[0..3] = {nblocks, qx, qy, qz}. */
+int mvd_get_geometry(mvd_session* s, int view, int* out);
+
+/* Thread-local message of the last failed call. */
+const char* mvd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MVDECON_H */

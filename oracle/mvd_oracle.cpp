@@ -1,0 +1,636 @@
+// Oracle: CPU restatement of the reference's efficient-Bayesian multiview
+// deconvolution (fiji/SPIM_Registration, Java).  TEST INFRASTRUCTURE ONLY:
+// it is loaded by tests/, by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg as the checker and the CPU baseline; the product library
+// never links or calls it.
+//
+// Two builds of the same templates: T = float restates the reference's FP32
+// path (float image data, double sums, the reference's cast points and
+// thread portioning); T = double is the FP64 twin used to report rounding
+// error.  Each function names the reference lines it restates.  Paths are
+// relative to src/main/java/ of the reference; deconv/ = spim/process/fusion/
+// deconvolution/, cuda/ = spim/process/cuda/.
+//
+// Third-party arithmetic that is not in the reference tree and is restated
+// from its published behaviour (parity unpinned at the rounding level):
+//  * imglib2-algorithm FFTConvolution (version managed by pom-scijava 34.1.0):
+//    image extended by the given out-of-bounds strategy, padded centred to a
+//    fast size >= n+k-1, kernel zero-padded and wrapped so its centre k/2
+//    sits at index 0, real FFT, complex product, inverse, centred unpad.
+//    Here the FFT is fft.hpp's and the fast sizes are 5-smooth.
+//  * imglib2 OutOfBoundsMirrorSingleBoundary: reflection without repeating
+//    the edge sample, period 2n-2.
+//  * mpicbg.util.RealSum (AdjustInput) and net.imglib2.util.RealSum
+//    (FirstIteration): restated as pairwise (binary-counter) and Kahan
+//    summation respectively.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fft.hpp"
+
+namespace orc {
+
+struct Dims {
+  int64_t x, y, z;
+  int64_t n() const { return x * y * z; }
+};
+
+enum Oob { OOB_MIRROR = 0, OOB_ONE = 1, OOB_ZERO = 2 };
+enum PsfType { OPTIMIZATION_II = 0, OPTIMIZATION_I = 1, EFFICIENT_BAYESIAN = 2, INDEPENDENT = 3 };
+
+// minValue, deconv/MVDeconvolution.java:70 (a float constant).
+static const float kMinValue = 0.0001f;
+
+// FusionHelper.divideIntoPortions, spim/process/fusion/FusionHelper.java:257-280:
+// k chunks of floor(n/k), the last one also takes the remainder.
+struct Portion {
+  int64_t start, size;
+};
+static std::vector<Portion> divide_into_portions(int64_t n, int k) {
+  std::vector<Portion> p;
+  const int64_t chunk = n / k, mod = n % k;
+  for (int i = 0; i < k; ++i) p.push_back({i * chunk, i == k - 1 ? chunk + mod : chunk});
+  return p;
+}
+
+// mpicbg.util.RealSum: binary-counter pairwise summation of doubles.
+class PairwiseSum {
+ public:
+  void add(double a) {
+    size_t i = 0;
+    double s = a;
+    while (i < flags_.size() && flags_[i]) {
+      flags_[i] = false;
+      s += sums_[i];
+      sums_[i] = 0.0;
+      ++i;
+    }
+    if (i == flags_.size()) {
+      flags_.push_back(false);
+      sums_.push_back(0.0);
+    }
+    flags_[i] = true;
+    sums_[i] = s;
+  }
+  double sum() const {
+    double s = 0;
+    for (double v : sums_) s += v;
+    return s;
+  }
+
+ private:
+  std::vector<bool> flags_;
+  std::vector<double> sums_;
+};
+
+// net.imglib2.util.RealSum: compensated (Kahan) summation.
+class KahanSum {
+ public:
+  void add(double v) {
+    const double y = v - c_;
+    const double t = s_ + y;
+    c_ = (t - s_) - y;
+    s_ = t;
+  }
+  double sum() const { return s_; }
+
+ private:
+  double s_ = 0, c_ = 0;
+};
+
+// AdjustInput.sumImg, deconv/AdjustInput.java:62-100.  2*numThreads portions;
+// the final sum adds sums[0] once before the loop over all portions, so
+// portion 0 is counted twice (AdjustInput.java:93-97).
+template <class T>
+double sum_img(const T* k, int64_t n, int num_threads) {
+  auto portions = divide_into_portions(n, 2 * num_threads);
+  std::vector<double> part;
+  for (auto& p : portions) {
+    PairwiseSum s;
+    for (int64_t j = 0; j < p.size; ++j) s.add((double)k[p.start + j]);
+    part.push_back(s.sum());
+  }
+  PairwiseSum total;
+  total.add(part[0]);
+  for (double v : part) total.add(v);
+  return total.sum();
+}
+
+// AdjustInput.normImg, deconv/AdjustInput.java:50-56: t = (T)((double)t / sum).
+template <class T>
+void norm_img(T* k, int64_t n, int num_threads) {
+  const double s = sum_img(k, n, num_threads);
+  for (int64_t i = 0; i < n; ++i) k[i] = (T)((double)k[i] / s);
+}
+
+// Mirror.mirror, deconv/Mirror.java:52-129, sequential semantics: every
+// position p <= n/2 along `axis` is swapped with n-1-p.  For even n the pair
+// (n/2-1, n/2) is swapped twice and therefore stays in place.
+template <class T>
+void mirror(T* img, Dims d, int axis) {
+  const int64_t len = axis == 0 ? d.x : axis == 1 ? d.y : d.z;
+  const int64_t half = len / 2, maxm = len - 1;
+  for (int64_t z = 0; z < d.z; ++z)
+    for (int64_t y = 0; y < d.y; ++y)
+      for (int64_t x = 0; x < d.x; ++x) {
+        int64_t p[3] = {x, y, z};
+        if (p[axis] > half) continue;
+        p[axis] = maxm - p[axis];
+        const int64_t a = x + d.x * (y + d.y * z);
+        const int64_t b = p[0] + d.x * (p[1] + d.y * p[2]);
+        std::swap(img[a], img[b]);
+      }
+}
+
+// MVDeconFFT.computeInvertedKernel, deconv/MVDeconFFT.java:336-344.
+template <class T>
+std::vector<T> inverted(const std::vector<T>& k, Dims d) {
+  std::vector<T> r = k;
+  for (int a = 0; a < 3; ++a) mirror(r.data(), d, a);
+  return r;
+}
+
+// imglib2 OutOfBoundsMirrorSingleBoundary index map.
+inline int64_t mirror_index(int64_t p, int64_t n) {
+  if (n == 1) return 0;
+  const int64_t period = 2 * n - 2;
+  p %= period;
+  if (p < 0) p += period;
+  return p < n ? p : period - p;
+}
+
+// Value of E(in) at integer position (x, y, z).
+template <class T>
+inline T extended(const T* in, Dims d, Oob oob, int64_t x, int64_t y, int64_t z) {
+  const bool inside = x >= 0 && x < d.x && y >= 0 && y < d.y && z >= 0 && z < d.z;
+  if (inside) return in[x + d.x * (y + d.y * z)];
+  if (oob == OOB_ONE) return (T)1;
+  if (oob == OOB_ZERO) return (T)0;
+  return in[mirror_index(x, d.x) + d.x * (mirror_index(y, d.y) + d.y * mirror_index(z, d.z))];
+}
+
+// FFTConvolution (imglib2-algorithm) as called by the reference:
+// out(x) = sum_j E(in)(x + k/2 - j) * ker(j), out has in's size.
+template <class T>
+void fft_convolve(const T* in, Dims din, Oob oob, const T* ker, Dims dk, T* out, int threads) {
+  using C = std::complex<T>;
+  const int64_t qx = next_fast_size(din.x + dk.x - 1, true);
+  const int64_t qy = next_fast_size(din.y + dk.y - 1, false);
+  const int64_t qz = next_fast_size(din.z + dk.z - 1, false);
+  const int64_t lx = (qx - din.x) / 2, ly = (qy - din.y) / 2, lz = (qz - din.z) / 2;
+  const int64_t q = qx * qy * qz;
+  std::vector<T> pad(q), kpad(q, (T)0);
+  parallel_for(qz, threads, [&](int64_t zb, int64_t ze) {
+    for (int64_t z = zb; z < ze; ++z)
+      for (int64_t y = 0; y < qy; ++y)
+        for (int64_t x = 0; x < qx; ++x)
+          pad[x + qx * (y + qy * z)] = extended(in, din, oob, x - lx, y - ly, z - lz);
+  });
+  const int64_t cx = dk.x / 2, cy = dk.y / 2, cz = dk.z / 2;
+  for (int64_t z = 0; z < dk.z; ++z)
+    for (int64_t y = 0; y < dk.y; ++y)
+      for (int64_t x = 0; x < dk.x; ++x) {
+        const int64_t hx = ((x - cx) % qx + qx) % qx, hy = ((y - cy) % qy + qy) % qy,
+                      hz = ((z - cz) % qz + qz) % qz;
+        kpad[hx + qx * (hy + qy * hz)] = ker[x + dk.x * (y + dk.y * z)];
+      }
+  RealFFT3D<T> fft((int)qx, (int)qy, (int)qz, threads);
+  std::vector<C> si(fft.spectrum_size()), sk(fft.spectrum_size());
+  fft.forward(pad.data(), si.data());
+  fft.forward(kpad.data(), sk.data());
+  parallel_for((int64_t)si.size(), threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const C a = si[i], h = sk[i];
+      si[i] = C(a.real() * h.real() - a.imag() * h.imag(), a.real() * h.imag() + a.imag() * h.real());
+    }
+  });
+  fft.inverse(si.data(), pad.data());
+  const T scale = (T)(1.0L / (long double)q);
+  parallel_for(din.z, threads, [&](int64_t zb, int64_t ze) {
+    for (int64_t z = zb; z < ze; ++z)
+      for (int64_t y = 0; y < din.y; ++y)
+        for (int64_t x = 0; x < din.x; ++x)
+          out[x + din.x * (y + din.y * z)] = pad[(x + lx) + qx * ((y + ly) + qy * (z + lz))] * scale;
+  });
+}
+
+// Brute-force FP64 direct convolution with the same definition.
+void direct_convolve(const double* in, Dims din, Oob oob, const double* ker, Dims dk, double* out,
+                     int threads) {
+  const int64_t cx = dk.x / 2, cy = dk.y / 2, cz = dk.z / 2;
+  parallel_for(din.z, threads, [&](int64_t zb, int64_t ze) {
+    for (int64_t z = zb; z < ze; ++z)
+      for (int64_t y = 0; y < din.y; ++y)
+        for (int64_t x = 0; x < din.x; ++x) {
+          double acc = 0;
+          for (int64_t j = 0; j < dk.z; ++j)
+            for (int64_t i = 0; i < dk.y; ++i)
+              for (int64_t h = 0; h < dk.x; ++h)
+                acc += extended(in, din, oob, x + cx - h, y + cy - i, z + cz - j) *
+                       ker[h + dk.x * (i + dk.y * j)];
+          out[x + din.x * (y + din.y * z)] = acc;
+        }
+  });
+}
+
+// BlockGeneratorFixedSizePrecise.divideIntoBlocks, cuda/BlockGeneratorFixedSizePrecise.java:46-122.
+struct BlockGeo {
+  int64_t offset[3], eff_size[3], eff_offset[3], eff_local[3];
+};
+static std::vector<BlockGeo> divide_into_blocks(Dims img, Dims ker, Dims bs) {
+  const int64_t is[3] = {img.x, img.y, img.z}, ks[3] = {ker.x, ker.y, ker.z}, b[3] = {bs.x, bs.y, bs.z};
+  int64_t eff[3], nb[3];
+  for (int d = 0; d < 3; ++d) {
+    eff[d] = b[d] - ks[d] + 1;
+    if (eff[d] <= 0) throw std::invalid_argument("block size smaller than kernel");
+    nb[d] = is[d] / eff[d] + (is[d] % eff[d] != 0);
+  }
+  std::vector<BlockGeo> out;
+  for (int64_t k = 0; k < nb[2]; ++k)
+    for (int64_t j = 0; j < nb[1]; ++j)
+      for (int64_t i = 0; i < nb[0]; ++i) {
+        const int64_t cur[3] = {i, j, k};
+        BlockGeo g;
+        for (int d = 0; d < 3; ++d) {
+          g.eff_size[d] = eff[d];
+          g.eff_offset[d] = cur[d] * eff[d];
+          g.offset[d] = g.eff_offset[d] - ks[d] / 2;
+          g.eff_local[d] = ks[d] / 2;
+          if (g.eff_offset[d] + g.eff_size[d] > is[d]) g.eff_size[d] = is[d] - g.eff_offset[d];
+        }
+        out.push_back(g);
+      }
+  return out;
+}
+
+// MVDeconFFT.convolve1/convolve2 on the CPU (deconv/MVDeconFFT.java:384-436,
+// 477-526; deconv/MVDeconFFTThreads.java:39-71; cuda/Block.java:134-215).
+// Without blocks the whole image is convolved with E = oob.  With blocks each
+// block is copied from E(in) (Block.copyBlock), convolved with the block's own
+// mirror-single extension (FFTConvolution(block, kernel)), and its effective
+// region pasted into out (Block.pasteBlock).
+template <class T>
+void convolve(const T* in, Dims d, Oob oob, const T* ker, Dims dk, const Dims* bs, T* out, int threads) {
+  if (!bs) {
+    fft_convolve(in, d, oob, ker, dk, out, threads);
+    return;
+  }
+  auto blocks = divide_into_blocks(d, dk, *bs);
+  std::vector<T> blk(bs->n()), res(bs->n());
+  for (auto& g : blocks) {
+    for (int64_t z = 0; z < bs->z; ++z)
+      for (int64_t y = 0; y < bs->y; ++y)
+        for (int64_t x = 0; x < bs->x; ++x)
+          blk[x + bs->x * (y + bs->y * z)] =
+              extended(in, d, oob, x + g.offset[0], y + g.offset[1], z + g.offset[2]);
+    fft_convolve(blk.data(), *bs, OOB_MIRROR, ker, dk, res.data(), threads);
+    for (int64_t z = 0; z < g.eff_size[2]; ++z)
+      for (int64_t y = 0; y < g.eff_size[1]; ++y)
+        for (int64_t x = 0; x < g.eff_size[0]; ++x)
+          out[(x + g.eff_offset[0]) + d.x * ((y + g.eff_offset[1]) + d.y * (z + g.eff_offset[2]))] =
+              res[(x + g.eff_local[0]) + bs->x * ((y + g.eff_local[1]) + bs->y * (z + g.eff_local[2]))];
+  }
+}
+
+// MVDeconFFT.init, deconv/MVDeconFFT.java:183-324, for all views in order
+// (MVDeconInput.init, deconv/MVDeconInput.java:62-68).  k1[v] is normalised in
+// place when view v is initialised; view v reads the current k1 of the other
+// views, which are normalised only if they come earlier.
+template <class T>
+void init_kernels(std::vector<std::vector<T>>& k1, const std::vector<Dims>& kd, PsfType type,
+                  int norm_threads, int threads, std::vector<std::vector<T>>& k2) {
+  const int nv = (int)k1.size();
+  k2.assign(nv, {});
+  for (int v = 0; v < nv; ++v) {
+    norm_img(k1[v].data(), kd[v].n(), norm_threads);
+    if (nv == 1 || type == INDEPENDENT) {
+      k2[v] = inverted(k1[v], kd[v]);
+    } else if (type == EFFICIENT_BAYESIAN) {
+      // MVDeconFFT.java:202-265
+      std::vector<T> tmp = inverted(k1[v], kd[v]);
+      std::vector<T> a(kd[v].n()), b(kd[v].n());
+      for (int w = 0; w < nv; ++w) {
+        if (w == v) continue;
+        std::vector<T> inv_v = inverted(k1[v], kd[v]);
+        fft_convolve(inv_v.data(), kd[v], OOB_ZERO, k1[w].data(), kd[w], a.data(), threads);
+        std::vector<T> inv_w = inverted(k1[w], kd[w]);
+        fft_convolve(a.data(), kd[v], OOB_ZERO, inv_w.data(), kd[w], b.data(), threads);
+        for (int64_t i = 0; i < kd[v].n(); ++i) tmp[i] = b[i] * tmp[i];
+      }
+      norm_img(tmp.data(), kd[v].n(), norm_threads);
+      k2[v] = tmp;
+    } else if (type == OPTIMIZATION_I) {
+      // MVDeconFFT.java:266-312
+      std::vector<T> tmp = k1[v];
+      std::vector<T> a(kd[v].n());
+      for (int w = 0; w < nv; ++w) {
+        if (w == v) continue;
+        std::vector<T> inv_w = inverted(k1[w], kd[w]);
+        fft_convolve(k1[v].data(), kd[v], OOB_ZERO, inv_w.data(), kd[w], a.data(), threads);
+        for (int64_t i = 0; i < kd[v].n(); ++i) tmp[i] = a[i] * tmp[i];
+      }
+      norm_img(tmp.data(), kd[v].n(), norm_threads);
+      k2[v] = inverted(tmp, kd[v]);
+    } else {
+      // OPTIMIZATION_II, MVDeconFFT.java:313-354: float power by repeated product.
+      std::vector<T> e = k1[v];
+      for (auto& f : e) {
+        T r = f;
+        for (int i = 1; i < nv; ++i) r *= f;
+        f = r;
+      }
+      norm_img(e.data(), kd[v].n(), norm_threads);
+      k2[v] = inverted(e, kd[v]);
+    }
+  }
+}
+
+// MVDeconvolution.fuseFirstIteration + FirstIteration.call, deconv/
+// MVDeconvolution.java:213-256 and deconv/FirstIteration.java:84-192.
+// Writes the per-voxel count of views with data into `counts` and returns
+// the average over voxels with data of the per-voxel mean.
+template <class T>
+double first_iteration(const std::vector<const T*>& imgs, int64_t n, int norm_threads, T* counts) {
+  auto portions = divide_into_portions(n, 2 * norm_threads);
+  KahanSum s;
+  int64_t count = 0;
+  for (auto& p : portions) {
+    KahanSum ps;
+    int64_t pc = 0;
+    for (int64_t i = p.start; i < p.start + p.size; ++i) {
+      double sum = 0;
+      int c = 0;
+      for (auto* im : imgs) {
+        const double v = (double)im[i];
+        if (v > 0) {
+          sum += v;
+          ++c;
+        }
+      }
+      if (c > 0) {
+        ps.add(sum / c);
+        ++pc;
+      }
+      counts[i] = (T)c;
+    }
+    s.add(ps.sum());
+    count += pc;
+  }
+  return s.sum() / (double)count;
+}
+
+// MVDeconvolution.computeQuotient, deconv/MVDeconvolution.java:494-546.
+template <class T>
+inline T quotient(T blurred, T img) {
+  return img > 0 ? img / blurred : (T)1;
+}
+
+// MVDeconvolution.computeNextValue + tikhonov, deconv/MVDeconvolution.java:692-726.
+template <class T>
+inline T next_value(T last, T integral, T weight, double lambda) {
+  const T minv = (T)kMinValue;
+  const T value = last * integral;
+  T adjusted;
+  if (value > 0) {
+    if (lambda > 0)
+      adjusted = (T)((std::sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
+    else
+      adjusted = value;
+  } else {
+    adjusted = minv;
+  }
+  T next;
+  if (std::isnan(adjusted))
+    next = minv;
+  else
+    next = std::max(minv, adjusted);
+  return last + ((next - last) * weight);
+}
+
+struct RunArgs {
+  int nviews;
+  Dims img;
+  std::vector<Dims> kdims;
+  PsfType type;
+  int iterations;
+  double lambda;
+  bool use_blocks;
+  Dims block;
+  int norm_threads;
+  int threads;
+};
+
+// MVDeconvolution.<init> + runIteration, deconv/MVDeconvolution.java:94-211,
+// 354-465.  stats receives, per iteration and view, {sum change, max change}.
+template <class T>
+void deconvolve(const RunArgs& a, const std::vector<const T*>& imgs, const std::vector<const T*>& weights,
+                std::vector<std::vector<T>> k1, T* psi, double* stats, const T* psi0,
+                std::vector<std::vector<T>>* k2_out) {
+  const int64_t n = a.img.n();
+  std::vector<std::vector<T>> k2;
+  init_kernels(k1, a.kdims, a.type, a.norm_threads, a.threads, k2);
+  if (k2_out) *k2_out = k2;
+
+  std::vector<T> tmp1(n), tmp2(n);
+  if (psi0) {
+    std::copy(psi0, psi0 + n, psi);
+  } else {
+    double avg = first_iteration(imgs, n, a.norm_threads, psi);
+    if (std::isnan(avg)) avg = 0.5;  // MVDeconvolution.java:138-142
+    const T v = (T)avg;
+    std::fill(psi, psi + n, v);
+  }
+  const Dims* bs = a.use_blocks ? &a.block : nullptr;
+  auto portions = divide_into_portions(n, 2 * a.norm_threads);
+  for (int it = 0; it < a.iterations; ++it) {
+    for (int v = 0; v < a.nviews; ++v) {
+      convolve(psi, a.img, OOB_MIRROR, k1[v].data(), a.kdims[v], bs, tmp1.data(), a.threads);
+      const T* img = imgs[v];
+      parallel_for(n, a.threads, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) tmp1[i] = quotient(tmp1[i], img[i]);
+      });
+      convolve(tmp1.data(), a.img, OOB_ONE, k2[v].data(), a.kdims[v], bs, tmp2.data(), a.threads);
+      const T* w = weights[v];
+      std::vector<double> psum(portions.size()), pmax(portions.size());
+      parallel_for((int64_t)portions.size(), a.threads, [&](int64_t pb, int64_t pe) {
+        for (int64_t p = pb; p < pe; ++p) {
+          double sum_change = 0, max_change = -1;
+          for (int64_t i = portions[p].start; i < portions[p].start + portions[p].size; ++i) {
+            const T last = psi[i];
+            const T next = next_value(last, tmp2[i], w ? w[i] : (T)1, a.lambda);
+            psi[i] = next;
+            const T change = std::abs(next - last);
+            sum_change += change;
+            max_change = std::max(max_change, (double)change);
+          }
+          psum[p] = sum_change;
+          pmax[p] = max_change;
+        }
+      });
+      double sum_change = 0, max_change = -1;
+      for (size_t p = 0; p < portions.size(); ++p) {
+        sum_change += psum[p];
+        max_change = std::max(max_change, pmax[p]);
+      }
+      if (stats) {
+        stats[(it * a.nviews + v) * 2 + 0] = sum_change;
+        stats[(it * a.nviews + v) * 2 + 1] = max_change;
+      }
+    }
+  }
+  // MVDeconvolution.java:201-208: zero where no view has data.
+  first_iteration(imgs, n, a.norm_threads, tmp1.data());
+  for (int64_t i = 0; i < n; ++i)
+    if (tmp1[i] == 0) psi[i] = 0;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+static thread_local std::string g_err;
+
+#define ORC_TRY(...)                \
+  try {                             \
+    __VA_ARGS__;                    \
+    return 0;                       \
+  } catch (const std::exception& e) { \
+    g_err = e.what();               \
+    return -1;                      \
+  }
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+// Portions of FusionHelper.divideIntoPortions: start[k], size[k].
+int orc_divide_portions(int64_t n, int k, int64_t* start, int64_t* size) {
+  ORC_TRY({
+    auto p = divide_into_portions(n, k);
+    for (int i = 0; i < k; ++i) {
+      start[i] = p[i].start;
+      size[i] = p[i].size;
+    }
+  })
+}
+
+// Block geometry: returns the number of blocks, fills up to `cap` records of
+// 12 int64 (offset[3], eff_size[3], eff_offset[3], eff_local[3]).
+int orc_divide_blocks(const int64_t* img, const int64_t* ker, const int64_t* bs, int64_t* out, int cap) {
+  try {
+    auto g = divide_into_blocks({img[0], img[1], img[2]}, {ker[0], ker[1], ker[2]}, {bs[0], bs[1], bs[2]});
+    for (int i = 0; i < (int)g.size() && i < cap; ++i)
+      for (int d = 0; d < 3; ++d) {
+        out[i * 12 + d] = g[i].offset[d];
+        out[i * 12 + 3 + d] = g[i].eff_size[d];
+        out[i * 12 + 6 + d] = g[i].eff_offset[d];
+        out[i * 12 + 9 + d] = g[i].eff_local[d];
+      }
+    return (int)g.size();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int orc_direct_convolve_f64(const double* in, const int64_t* dims, int oob, const double* ker,
+                            const int64_t* kdims, double* out, int threads) {
+  ORC_TRY(direct_convolve(in, {dims[0], dims[1], dims[2]}, (Oob)oob, ker, {kdims[0], kdims[1], kdims[2]}, out,
+                          threads))
+}
+
+#define ORC_TYPED(T, SUF)                                                                              \
+  double orc_sum_img_##SUF(const T* k, int64_t n, int norm_threads) {                                 \
+    return sum_img(k, n, norm_threads);                                                                \
+  }                                                                                                    \
+  int orc_norm_img_##SUF(T* k, int64_t n, int norm_threads) { ORC_TRY(norm_img(k, n, norm_threads)) } \
+  int orc_mirror_##SUF(T* img, const int64_t* dims, int axis) {                                       \
+    ORC_TRY(mirror(img, {dims[0], dims[1], dims[2]}, axis))                                           \
+  }                                                                                                    \
+  int orc_fft_convolve_##SUF(const T* in, const int64_t* dims, int oob, const T* ker, const int64_t* kdims, \
+                             T* out, int threads) {                                                    \
+    ORC_TRY(fft_convolve(in, {dims[0], dims[1], dims[2]}, (Oob)oob, ker, {kdims[0], kdims[1], kdims[2]}, \
+                         out, threads))                                                                \
+  }                                                                                                    \
+  int orc_convolve_##SUF(const T* in, const int64_t* dims, int oob, const T* ker, const int64_t* kdims, \
+                         const int64_t* block, T* out, int threads) {                                  \
+    ORC_TRY({                                                                                          \
+      Dims bs{0, 0, 0};                                                                                \
+      if (block) bs = {block[0], block[1], block[2]};                                                  \
+      convolve(in, {dims[0], dims[1], dims[2]}, (Oob)oob, ker, {kdims[0], kdims[1], kdims[2]},         \
+               block ? &bs : nullptr, out, threads);                                                   \
+    })                                                                                                 \
+  }                                                                                                    \
+  /* kernels: nviews arrays of kdims[3*v..] each; k2 receives kernel2 */                               \
+  int orc_init_kernels_##SUF(int nviews, T* const* k1, const int64_t* kdims, int psf_type,            \
+                             int norm_threads, int threads, T* const* k2) {                            \
+    ORC_TRY({                                                                                          \
+      std::vector<std::vector<T>> kk(nviews), out;                                                     \
+      std::vector<Dims> kd(nviews);                                                                    \
+      for (int v = 0; v < nviews; ++v) {                                                               \
+        kd[v] = {kdims[3 * v], kdims[3 * v + 1], kdims[3 * v + 2]};                                    \
+        kk[v].assign(k1[v], k1[v] + kd[v].n());                                                        \
+      }                                                                                                \
+      init_kernels(kk, kd, (PsfType)psf_type, norm_threads, threads, out);                             \
+      for (int v = 0; v < nviews; ++v) {                                                               \
+        std::copy(kk[v].begin(), kk[v].end(), k1[v]);                                                  \
+        std::copy(out[v].begin(), out[v].end(), k2[v]);                                                \
+      }                                                                                                \
+    })                                                                                                 \
+  }                                                                                                    \
+  double orc_first_iteration_##SUF(int nviews, const T* const* imgs, int64_t n, int norm_threads,     \
+                                   T* counts) {                                                        \
+    std::vector<const T*> v(imgs, imgs + nviews);                                                      \
+    return first_iteration(v, n, norm_threads, counts);                                                \
+  }                                                                                                    \
+  int orc_quotient_##SUF(T* blurred, const T* img, int64_t n) {                                       \
+    for (int64_t i = 0; i < n; ++i) blurred[i] = quotient(blurred[i], img[i]);                         \
+    return 0;                                                                                          \
+  }                                                                                                    \
+  int orc_next_value_##SUF(const T* last, const T* integral, const T* weight, int64_t n, double lambda, \
+                           T* out) {                                                                   \
+    for (int64_t i = 0; i < n; ++i) out[i] = next_value(last[i], integral[i], weight[i], lambda);      \
+    return 0;                                                                                          \
+  }                                                                                                    \
+  int orc_deconvolve_##SUF(int nviews, const int64_t* dims, const T* const* imgs, const T* const* weights, \
+                           const T* const* kernels, const int64_t* kdims, int psf_type, int iterations, \
+                           double lambda, const int64_t* block, int norm_threads, int threads,         \
+                           const T* psi0, T* psi, double* stats, T* const* k2_out) {                   \
+    ORC_TRY({                                                                                          \
+      RunArgs a;                                                                                       \
+      a.nviews = nviews;                                                                               \
+      a.img = {dims[0], dims[1], dims[2]};                                                             \
+      std::vector<std::vector<T>> k1(nviews);                                                          \
+      for (int v = 0; v < nviews; ++v) {                                                               \
+        a.kdims.push_back({kdims[3 * v], kdims[3 * v + 1], kdims[3 * v + 2]});                         \
+        k1[v].assign(kernels[v], kernels[v] + a.kdims[v].n());                                         \
+      }                                                                                                \
+      a.type = (PsfType)psf_type;                                                                      \
+      a.iterations = iterations;                                                                       \
+      a.lambda = lambda;                                                                               \
+      a.use_blocks = block != nullptr;                                                                 \
+      a.block = block ? Dims{block[0], block[1], block[2]} : Dims{0, 0, 0};                            \
+      a.norm_threads = norm_threads;                                                                   \
+      a.threads = threads;                                                                             \
+      std::vector<const T*> im(imgs, imgs + nviews), w(nviews, nullptr);                               \
+      if (weights)                                                                                     \
+        for (int v = 0; v < nviews; ++v) w[v] = weights[v];                                            \
+      std::vector<std::vector<T>> k2;                                                                  \
+      deconvolve(a, im, w, k1, psi, stats, psi0, k2_out ? &k2 : nullptr);                              \
+      if (k2_out)                                                                                      \
+        for (int v = 0; v < nviews; ++v) std::copy(k2[v].begin(), k2[v].end(), k2_out[v]);             \
+    })                                                                                                 \
+  }
+
+ORC_TYPED(float, f32)
+ORC_TYPED(double, f64)
+
+}  // extern "C"

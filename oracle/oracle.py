@@ -1,0 +1,242 @@
+"""ctypes wrapper of the oracle (oracle/mvd_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is a CPU restatement of the reference's
+float32 deconvolution path plus an FP64 twin.  Only tests/, the smoke check
+and the benchmark's CPU-baseline leg import this module, and only as the
+checker or the baseline; the product library never calls it.
+
+Arrays are numpy, x-fastest, indexed [z, y, x].  Dimension tuples passed to
+the C functions are (nx, ny, nz).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmvd_oracle.so")
+
+OOB_MIRROR, OOB_ONE, OOB_ZERO = 0, 1, 2
+OPTIMIZATION_II, OPTIMIZATION_I, EFFICIENT_BAYESIAN, INDEPENDENT = 0, 1, 2, 3
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile; returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+def _declare(L):
+    L.orc_last_error.restype = ctypes.c_char_p
+    L.orc_divide_portions.argtypes = [ctypes.c_int64, ctypes.c_int, _P, _P]
+    L.orc_divide_blocks.argtypes = [_P, _P, _P, _P, ctypes.c_int]
+    L.orc_direct_convolve_f64.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]
+    for suf in ("f32", "f64"):
+        getattr(L, f"orc_sum_img_{suf}").argtypes = [_P, ctypes.c_int64, ctypes.c_int]
+        getattr(L, f"orc_sum_img_{suf}").restype = ctypes.c_double
+        getattr(L, f"orc_norm_img_{suf}").argtypes = [_P, ctypes.c_int64, ctypes.c_int]
+        getattr(L, f"orc_mirror_{suf}").argtypes = [_P, _P, ctypes.c_int]
+        getattr(L, f"orc_fft_convolve_{suf}").argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int]
+        getattr(L, f"orc_convolve_{suf}").argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P, ctypes.c_int]
+        getattr(L, f"orc_init_kernels_{suf}").argtypes = [
+            ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+        getattr(L, f"orc_first_iteration_{suf}").argtypes = [ctypes.c_int, _P, ctypes.c_int64, ctypes.c_int, _P]
+        getattr(L, f"orc_first_iteration_{suf}").restype = ctypes.c_double
+        getattr(L, f"orc_quotient_{suf}").argtypes = [_P, _P, ctypes.c_int64]
+        getattr(L, f"orc_next_value_{suf}").argtypes = [_P, _P, _P, ctypes.c_int64, ctypes.c_double, _P]
+        getattr(L, f"orc_deconvolve_{suf}").argtypes = [
+            ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _P,
+            ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def _dt(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return "f32"
+    if dtype == np.float64:
+        return "f64"
+    raise TypeError(f"oracle supports float32/float64, got {dtype}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _dims(a):
+    """(nx, ny, nz) of a [z, y, x] array as an int64 array."""
+    z, y, x = a.shape
+    return np.array([x, y, z], dtype=np.int64)
+
+
+def _ptr_array(arrays):
+    return (_P * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def divide_portions(n: int, k: int):
+    s = np.zeros(k, np.int64)
+    z = np.zeros(k, np.int64)
+    _check(lib().orc_divide_portions(n, k, _ptr(s), _ptr(z)))
+    return list(zip(s.tolist(), z.tolist()))
+
+
+def divide_blocks(img_xyz, ker_xyz, block_xyz):
+    """List of dicts with offset/eff_size/eff_offset/eff_local (x, y, z)."""
+    i = np.asarray(img_xyz, np.int64)
+    k = np.asarray(ker_xyz, np.int64)
+    b = np.asarray(block_xyz, np.int64)
+    cap = 4096
+    out = np.zeros(cap * 12, np.int64)
+    nb = lib().orc_divide_blocks(_ptr(i), _ptr(k), _ptr(b), _ptr(out), cap)
+    if nb < 0:
+        raise ValueError(lib().orc_last_error().decode())
+    res = []
+    for j in range(nb):
+        r = out[j * 12:(j + 1) * 12]
+        res.append(dict(offset=tuple(r[0:3]), eff_size=tuple(r[3:6]),
+                        eff_offset=tuple(r[6:9]), eff_local=tuple(r[9:12])))
+    return res
+
+
+def sum_img(k, norm_threads: int) -> float:
+    k = np.ascontiguousarray(k)
+    return getattr(lib(), f"orc_sum_img_{_dt(k.dtype)}")(_ptr(k), k.size, norm_threads)
+
+
+def norm_img(k, norm_threads: int):
+    k = np.array(k, copy=True, order="C")
+    _check(getattr(lib(), f"orc_norm_img_{_dt(k.dtype)}")(_ptr(k), k.size, norm_threads))
+    return k
+
+
+def mirror(img, axis_xyz: int):
+    img = np.array(img, copy=True, order="C")
+    _check(getattr(lib(), f"orc_mirror_{_dt(img.dtype)}")(_ptr(img), _ptr(_dims(img)), axis_xyz))
+    return img
+
+
+def fft_convolve(img, kernel, oob: int, threads: int | None = None):
+    img = np.ascontiguousarray(img)
+    kernel = np.ascontiguousarray(kernel, dtype=img.dtype)
+    out = np.empty_like(img)
+    _check(getattr(lib(), f"orc_fft_convolve_{_dt(img.dtype)}")(
+        _ptr(img), _ptr(_dims(img)), oob, _ptr(kernel), _ptr(_dims(kernel)), _ptr(out),
+        threads or default_threads()))
+    return out
+
+
+def convolve(img, kernel, oob: int, block_xyz=None, threads: int | None = None):
+    img = np.ascontiguousarray(img)
+    kernel = np.ascontiguousarray(kernel, dtype=img.dtype)
+    out = np.empty_like(img)
+    blk = None if block_xyz is None else np.asarray(block_xyz, np.int64)
+    _check(getattr(lib(), f"orc_convolve_{_dt(img.dtype)}")(
+        _ptr(img), _ptr(_dims(img)), oob, _ptr(kernel), _ptr(_dims(kernel)),
+        None if blk is None else _ptr(blk), _ptr(out), threads or default_threads()))
+    return out
+
+
+def direct_convolve_f64(img, kernel, oob: int, threads: int | None = None):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    kernel = np.ascontiguousarray(kernel, dtype=np.float64)
+    out = np.empty_like(img)
+    _check(lib().orc_direct_convolve_f64(_ptr(img), _ptr(_dims(img)), oob, _ptr(kernel),
+                                         _ptr(_dims(kernel)), _ptr(out), threads or default_threads()))
+    return out
+
+
+def init_kernels(kernels, psf_type: int, norm_threads: int, threads: int | None = None):
+    """Returns (kernel1 normalised, kernel2) lists, as MVDeconFFT.init does."""
+    dtype = kernels[0].dtype
+    k1 = [np.array(k, copy=True, order="C", dtype=dtype) for k in kernels]
+    k2 = [np.empty_like(k) for k in k1]
+    kd = np.concatenate([_dims(k) for k in k1]).astype(np.int64)
+    _check(getattr(lib(), f"orc_init_kernels_{_dt(dtype)}")(
+        len(k1), _ptr_array(k1), _ptr(kd), psf_type, norm_threads, threads or default_threads(),
+        _ptr_array(k2)))
+    return k1, k2
+
+
+def first_iteration(imgs, norm_threads: int):
+    """Returns (average, counts image)."""
+    imgs = [np.ascontiguousarray(i) for i in imgs]
+    counts = np.empty_like(imgs[0])
+    avg = getattr(lib(), f"orc_first_iteration_{_dt(imgs[0].dtype)}")(
+        len(imgs), _ptr_array(imgs), imgs[0].size, norm_threads, _ptr(counts))
+    return avg, counts
+
+
+def quotient(blurred, img):
+    b = np.array(blurred, copy=True, order="C")
+    img = np.ascontiguousarray(img, dtype=b.dtype)
+    _check(getattr(lib(), f"orc_quotient_{_dt(b.dtype)}")(_ptr(b), _ptr(img), b.size))
+    return b
+
+
+def next_value(last, integral, weight, lam: float):
+    last = np.ascontiguousarray(last)
+    integral = np.ascontiguousarray(integral, dtype=last.dtype)
+    weight = np.ascontiguousarray(weight, dtype=last.dtype)
+    out = np.empty_like(last)
+    _check(getattr(lib(), f"orc_next_value_{_dt(last.dtype)}")(
+        _ptr(last), _ptr(integral), _ptr(weight), last.size, lam, _ptr(out)))
+    return out
+
+
+def deconvolve(views, kernels, psf_type: int, iterations: int, lam: float, weights=None,
+               block_xyz=None, norm_threads: int = 8, threads: int | None = None, psi0=None,
+               return_kernels: bool = False):
+    """Full MVDeconvolution run.  Returns (psi, stats[iterations, views, 2]) and
+    optionally kernel2 per view."""
+    dtype = views[0].dtype
+    views = [np.ascontiguousarray(v, dtype=dtype) for v in views]
+    kernels = [np.ascontiguousarray(k, dtype=dtype) for k in kernels]
+    nv = len(views)
+    dims = _dims(views[0])
+    kd = np.concatenate([_dims(k) for k in kernels]).astype(np.int64)
+    w_arr = None
+    if weights is not None:
+        weights = [np.ascontiguousarray(w, dtype=dtype) for w in weights]
+        w_arr = _ptr_array(weights)
+    psi = np.empty_like(views[0])
+    stats = np.zeros((iterations, nv, 2), np.float64)
+    blk = None if block_xyz is None else np.asarray(block_xyz, np.int64)
+    k2 = [np.empty_like(k) for k in kernels] if return_kernels else None
+    p0 = None
+    if psi0 is not None:
+        p0 = np.ascontiguousarray(psi0, dtype=dtype)
+    _check(getattr(lib(), f"orc_deconvolve_{_dt(dtype)}")(
+        nv, _ptr(dims), _ptr_array(views), w_arr, _ptr_array(kernels), _ptr(kd), psf_type,
+        iterations, lam, None if blk is None else _ptr(blk), norm_threads, threads or default_threads(),
+        None if p0 is None else _ptr(p0), _ptr(psi), _ptr(stats),
+        _ptr_array(k2) if k2 is not None else None))
+    if return_kernels:
+        return psi, stats, k2
+    return psi, stats

@@ -1,0 +1,86 @@
+"""ctypes bindings of libmvdecon_hip.so (declared in include/mvdecon.h).
+
+The library is the product path: there is no CPU fallback.  If it has not
+been built, or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmvdecon_hip.so")
+
+_P = ctypes.c_void_p
+_lib = None
+
+# Every symbol declared in include/mvdecon.h.
+EXPORTS = (
+    "getNumDevicesCUDA", "getNameDeviceCUDA", "getMemDeviceCUDA", "getFreeMemDeviceCUDA",
+    "getCUDAcomputeCapabilityMajorVersion", "getCUDAcomputeCapabilityMinorVersion",
+    "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
+    "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
+    "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
+    "mvd_timing_name", "mvd_get_geometry", "mvd_last_error",
+)
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def _declare(L):
+    L.getNumDevicesCUDA.restype = ctypes.c_int
+    L.getNameDeviceCUDA.argtypes = [ctypes.c_int, ctypes.c_char_p]
+    L.getNameDeviceCUDA.restype = None
+    L.getMemDeviceCUDA.argtypes = [ctypes.c_int]
+    L.getMemDeviceCUDA.restype = ctypes.c_longlong
+    L.getFreeMemDeviceCUDA.argtypes = [ctypes.c_int]
+    L.getFreeMemDeviceCUDA.restype = ctypes.c_longlong
+    L.getCUDAcomputeCapabilityMajorVersion.argtypes = [ctypes.c_int]
+    L.getCUDAcomputeCapabilityMinorVersion.argtypes = [ctypes.c_int]
+    L.convolution3DfftCUDAInPlace.argtypes = [_P, _P, _P, _P, ctypes.c_int]
+    L.convolution3DfftCUDAInPlace.restype = None
+    L.convolution3DfftCUDA.argtypes = [_P, _P, _P, _P, ctypes.c_int]
+    L.convolution3DfftCUDA.restype = ctypes.POINTER(ctypes.c_float)
+    L.mvd_free.argtypes = [_P]
+    L.mvd_free.restype = None
+    L.mvd_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
+    L.mvd_destroy.argtypes = [_P]
+    L.mvd_add_view.argtypes = [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.mvd_set_weights.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _P]
+    L.mvd_prepare.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, ctypes.c_int]
+    L.mvd_init_psi.argtypes = [_P, _P, _P]
+    L.mvd_iterate.argtypes = [_P, ctypes.c_int, _P]
+    L.mvd_get_psi.argtypes = [_P, _P, ctypes.c_int]
+    L.mvd_get_kernels.argtypes = [_P, ctypes.c_int, _P, _P]
+    L.mvd_set_psf.argtypes = [_P, ctypes.c_int, _P]
+    L.mvd_set_timing.argtypes = [_P, ctypes.c_int]
+    L.mvd_get_timing.argtypes = [_P, _P, _P, ctypes.c_int]
+    L.mvd_timing_name.argtypes = [ctypes.c_int]
+    L.mvd_timing_name.restype = ctypes.c_char_p
+    L.mvd_get_geometry.argtypes = [_P, ctypes.c_int, _P]
+    L.mvd_last_error.restype = ctypes.c_char_p
+
+
+def lib():
+    """The loaded library; raises NativeError if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(
+                f"{LIB_PATH} is missing: build it with `python -m spim_registration_amd.build` "
+                "(there is no CPU fallback)")
+        try:
+            L = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise NativeError(f"cannot load {LIB_PATH}: {e}") from e
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().mvd_last_error().decode(errors="replace")
+        raise NativeError(f"{what}: {msg} (code {rc})")

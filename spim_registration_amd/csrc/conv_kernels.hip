@@ -1,0 +1,401 @@
+// FFT convolution passes of one block (window) of a volume, for gfx950.
+//
+// A 3D real convolution with circular wrap over the padded size q is done in
+// four launches over a half spectrum S laid out [z][y][kx], kx in [0, q.x/2]:
+//   x_forward : gather the window (with the out-of-bounds rule of the
+//               reference), real-to-complex transform along x (a length q.x/2
+//               complex FFT of the packed even/odd samples plus an unpack
+//               step), write S rows.
+//   y         : complex FFT along y of every (z, kx) column.
+//   z         : forward FFT along z, product with the kernel spectrum,
+//               inverse FFT along z -- one launch, the line never leaves LDS.
+//               Only the z rows the output box needs are written back.
+//   y (inv)   : inverse FFT along y, only for the needed z, and only the
+//               needed y rows are written.
+//   x_inverse : complex-to-real along x for the output rows, then the fused
+//               epilogue (store, quotient or multiplicative update).
+// Kernel spectra are pre-scaled by 1/(q.x q.y q.z), so every transform here
+// is unnormalised.
+#include "mvd_internal.h"
+
+namespace mvd {
+
+__device__ __forceinline__ int mirror_idx(int p, int n) {
+  if (p >= 0 && p < n) return p;
+  if (n == 1) return 0;
+  const int period = 2 * n - 2;
+  p %= period;
+  if (p < 0) p += period;
+  return p < n ? p : period - p;
+}
+
+struct XFwdParams {
+  ConvGeom g;
+  const float* src;
+  int gather;
+  float2* spec;
+  FftPlan plan;
+  const float2* tw;
+  int hx;
+  int lines;
+};
+
+__device__ __forceinline__ float gather_value(const XFwdParams& P, int x, int y, int z) {
+  const ConvGeom& g = P.g;
+  if (P.gather == kGatherKernelWrap) {
+    // padded position (x, y, z) holds k[(p + k/2) mod q] when that index lies in the kernel
+    int jx = x + g.src.x / 2, jy = y + g.src.y / 2, jz = z + g.src.z / 2;
+    if (jx >= g.q.x) jx -= g.q.x;
+    if (jy >= g.q.y) jy -= g.q.y;
+    if (jz >= g.q.z) jz -= g.q.z;
+    if (jx < g.src.x && jy < g.src.y && jz < g.src.z)
+      return P.src[jx + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz)];
+    return 0.f;
+  }
+  const int gx = g.off.x + x, gy = g.off.y + y, gz = g.off.z + z;
+  const bool inside = gx >= 0 && gx < g.src.x && gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
+  if (inside) return P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+  if (P.gather == kGatherOne) return 1.f;
+  if (P.gather == kGatherZero) return 0.f;
+  const int mx = mirror_idx(gx, g.src.x), my = mirror_idx(gy, g.src.y), mz = mirror_idx(gz, g.src.z);
+  return P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
+}
+
+__global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
+  extern __shared__ float2 smem[];
+  const int qx = P.g.q.x, M = qx / 2;
+  const int B = P.lines, lp = B + 1;
+  float2* tw = smem;
+  float2* buf = smem + qx;
+  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
+  const int nrows = P.g.q.y * P.g.q.z;
+  const int row0 = blockIdx.x * B;
+  float* fbuf = reinterpret_cast<float*>(buf);
+  for (int idx = threadIdx.x; idx < B * qx; idx += kThreads) {
+    const int l = idx / qx, x = idx - l * qx;
+    const int row = row0 + l;
+    float v = 0.f;
+    if (row < nrows) {
+      const int y = row % P.g.q.y, z = row / P.g.q.y;
+      if (x < P.g.win.x && y < P.g.win.y && z < P.g.win.z) v = gather_value(P, x, y, z);
+    }
+    fbuf[2 * ((x >> 1) * lp + l) + (x & 1)] = v;
+  }
+  __syncthreads();
+  lds_fft(buf, lp, B, P.plan, tw, 2, -1.f);
+  // unpack Z = FFT(even + i odd) into X[k], k in [0, M]; thread handles the pair (k, M - k)
+  const int K2 = M / 2 + 1;
+  for (int idx = threadIdx.x; idx < B * K2; idx += kThreads) {
+    const int l = idx / K2, k = idx - l * K2;
+    const int row = row0 + l;
+    if (row >= nrows) continue;
+    const int km = (M - k) % M;
+    const float2 a = buf[k * lp + l], b = buf[km * lp + l];
+    float2* out = P.spec + (int64_t)row * P.hx;
+    {
+      const float2 bc = cconj(b);
+      const float2 E = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
+      const float2 O = make_float2(D.y, -D.x);
+      out[k] = cadd(E, cmul(tw[k], O));
+    }
+    if (M - k != k) {
+      const float2 ac = cconj(a);
+      const float2 E = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
+      const float2 O = make_float2(D.y, -D.x);
+      out[M - k] = cadd(E, cmul(tw[M - k], O));
+    }
+  }
+}
+
+struct YParams {
+  float2* spec;
+  int hx, qy, zlo, ylo, yhi, lines;
+  float dir;
+  FftPlan plan;
+  const float2* tw;
+};
+
+__global__ __launch_bounds__(kThreads) void k_y(YParams P) {
+  extern __shared__ float2 smem[];
+  const int B = P.lines, lp = B + 1, qy = P.qy;
+  float2* tw = smem;
+  float2* buf = smem + qy;
+  for (int i = threadIdx.x; i < qy; i += kThreads) tw[i] = P.tw[i];
+  const int c0 = blockIdx.x * B;
+  const int z = P.zlo + blockIdx.y;
+  float2* base = P.spec + (int64_t)z * qy * P.hx + c0;
+  for (int idx = threadIdx.x; idx < qy * B; idx += kThreads) {
+    const int y = idx / B, b = idx - y * B;
+    buf[y * lp + b] = (c0 + b < P.hx) ? base[(int64_t)y * P.hx + b] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft(buf, lp, B, P.plan, tw, 1, P.dir);
+  const int nrow = P.yhi - P.ylo;
+  for (int idx = threadIdx.x; idx < nrow * B; idx += kThreads) {
+    const int y = P.ylo + idx / B, b = idx % B;
+    if (c0 + b < P.hx) base[(int64_t)y * P.hx + b] = buf[y * lp + b];
+  }
+}
+
+struct ZParams {
+  float2* spec;
+  const float2* kspec;
+  float scale;
+  int hx, qy, qz, zlo, zhi, lines;
+  FftPlan plan;
+  const float2* tw;
+};
+
+__global__ __launch_bounds__(kThreads) void k_z(ZParams P) {
+  extern __shared__ float2 smem[];
+  const int B = P.lines, lp = B + 1, qz = P.qz;
+  float2* tw = smem;
+  float2* buf = smem + qz;
+  for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = P.tw[i];
+  const int c0 = blockIdx.x * B;
+  const int y = blockIdx.y;
+  const int64_t plane = (int64_t)P.qy * P.hx;
+  const int64_t col = (int64_t)y * P.hx + c0;
+  for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
+    const int z = idx / B, b = idx - z * B;
+    buf[z * lp + b] = (c0 + b < P.hx) ? P.spec[z * plane + col + b] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft(buf, lp, B, P.plan, tw, 1, -1.f);
+  if (P.kspec) {
+    for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
+      const int z = idx / B, b = idx - z * B;
+      if (c0 + b < P.hx) buf[z * lp + b] = cmul(buf[z * lp + b], P.kspec[z * plane + col + b]);
+    }
+    __syncthreads();
+    lds_fft(buf, lp, B, P.plan, tw, 1, 1.f);
+    const int nrow = P.zhi - P.zlo;
+    for (int idx = threadIdx.x; idx < nrow * B; idx += kThreads) {
+      const int z = P.zlo + idx / B, b = idx % B;
+      if (c0 + b < P.hx) P.spec[z * plane + col + b] = buf[z * lp + b];
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
+      const int z = idx / B, b = idx - z * B;
+      if (c0 + b < P.hx) P.spec[z * plane + col + b] = cscale(buf[z * lp + b], P.scale);
+    }
+  }
+}
+
+// MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726),
+// evaluated with the reference's float/double cast points and no contraction.
+__device__ __forceinline__ float next_value(float last, float integral, float weight, double lambda) {
+#pragma clang fp contract(off)
+  const float minv = 0.0001f;
+  const float value = last * integral;
+  float adjusted;
+  if (value > 0.f) {
+    if (lambda > 0.0)
+      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
+    else
+      adjusted = value;
+  } else {
+    adjusted = minv;
+  }
+  const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
+  return last + ((next - last) * weight);
+}
+
+__device__ __forceinline__ float quotient(float blurred, float img) {
+#pragma clang fp contract(off)
+  return img > 0.f ? img / blurred : 1.f;
+}
+
+struct XInvParams {
+  ConvGeom g;
+  const float2* spec;
+  EpilogueArgs e;
+  FftPlan plan;
+  const float2* tw;
+  int hx, lines;
+};
+
+__global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
+  extern __shared__ float2 smem[];
+  __shared__ double red_sum[kThreads / 64];
+  __shared__ float red_max[kThreads / 64];
+  const ConvGeom& g = P.g;
+  const int qx = g.q.x, M = qx / 2;
+  const int B = P.lines, lp = B + 1;
+  float2* tw = smem;
+  float2* buf = smem + qx;
+  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
+  const int ny = g.eff_size.y;
+  const int nrows = ny * g.eff_size.z;
+  const int row0 = blockIdx.x * B;
+  const int H = M + 1;
+  for (int idx = threadIdx.x; idx < B * H; idx += kThreads) {
+    const int l = idx / H, k = idx - l * H;
+    const int row = row0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
+      v = P.spec[((int64_t)z * g.q.y + y) * P.hx + k];
+    }
+    buf[k * lp + l] = v;
+  }
+  __syncthreads();
+  // pack X[k], X[M-k] into Z[k] = E + i O (even/odd inverse split)
+  const int K2 = M / 2 + 1;
+  for (int idx = threadIdx.x; idx < B * K2; idx += kThreads) {
+    const int l = idx / K2, k = idx - l * K2;
+    const float2 a = buf[k * lp + l], b = buf[(M - k) * lp + l];
+    const float2 za = cadd(cadd(a, cconj(b)), mul_di(cmul(csub(a, cconj(b)), cconj(tw[k])), 1.f));
+    if (k != 0 && M - k != k) {
+      const float2 zb = cadd(cadd(b, cconj(a)), mul_di(cmul(csub(b, cconj(a)), cconj(tw[M - k])), 1.f));
+      buf[(M - k) * lp + l] = zb;
+    }
+    buf[k * lp + l] = za;
+  }
+  __syncthreads();
+  lds_fft(buf, lp, B, P.plan, tw, 2, 1.f);
+  const float* fbuf = reinterpret_cast<const float*>(buf);
+  const EpilogueArgs& e = P.e;
+  double my_sum = 0.0;
+  float my_max = -1.f;
+  const int nx = g.eff_size.x;
+  for (int idx = threadIdx.x; idx < B * nx; idx += kThreads) {
+    const int l = idx / nx, xo = idx - l * nx;
+    const int row = row0 + l;
+    if (row >= nrows) continue;
+    const int x = g.eff_local.x + xo;
+    const float v = fbuf[2 * ((x >> 1) * lp + l) + (x & 1)];
+    const int yo = row % ny, zo = row / ny;
+    const int64_t gi = (int64_t)(g.eff_offset.x + xo) +
+                       (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
+    if (e.mode == kEpiStore) {
+      e.dst[gi] = v;
+    } else if (e.mode == kEpiQuotient) {
+      e.dst[gi] = quotient(v, e.img[gi]);
+    } else {
+      float w = 1.f;
+      if (e.weight_mode == kWeightArray) {
+        w = e.weight[gi];
+      } else if (e.weight_mode == kWeightVirtual) {
+        const double r = (double)e.weight[gi] / (double)e.wsum[gi];
+        w = (float)fmin(1.0, r * e.osem);
+      }
+      const float last = e.dst[gi];
+      const float next = next_value(last, v, w, e.lambda);
+      e.dst[gi] = next;
+      const float change = fabsf(next - last);
+      my_sum += (double)change;
+      my_max = fmaxf(my_max, change);
+    }
+  }
+  if (e.mode == kEpiUpdate) {
+    for (int o = 32; o > 0; o >>= 1) {
+      my_sum += __shfl_xor(my_sum, o);
+      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+    }
+    const int wid = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+      red_sum[wid] = my_sum;
+      red_max[wid] = my_max;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      float m = -1.f;
+      for (int i = 0; i < kThreads / 64; ++i) {
+        s += red_sum[i];
+        m = fmaxf(m, red_max[i]);
+      }
+      e.part_sum[blockIdx.x] = s;
+      e.part_max[blockIdx.x] = m;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                            hipStream_t stream) {
+  XFwdParams P;
+  P.g = g;
+  P.src = src;
+  P.gather = gather;
+  P.spec = spec;
+  P.plan = sp.px;
+  P.tw = sp.tw_x;
+  P.hx = sp.hx;
+  P.lines = sp.bx;
+  const int nrows = g.q.y * g.q.z;
+  const int M = g.q.x / 2;
+  dim3 grid((nrows + P.lines - 1) / P.lines);
+  size_t lds = sizeof(float2) * (g.q.x + (size_t)M * (P.lines + 1));
+  hipLaunchKernelGGL(k_x_forward, grid, dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                    hipStream_t stream) {
+  if (zhi <= zlo) return hipSuccess;
+  YParams P;
+  P.spec = spec;
+  P.hx = sp.hx;
+  P.qy = sp.q.y;
+  P.zlo = zlo;
+  P.ylo = ylo;
+  P.yhi = yhi;
+  P.lines = sp.by;
+  P.dir = dir;
+  P.plan = sp.py;
+  P.tw = sp.tw_y;
+  dim3 grid((sp.hx + P.lines - 1) / P.lines, zhi - zlo);
+  size_t lds = sizeof(float2) * (sp.q.y + (size_t)sp.q.y * (P.lines + 1));
+  hipLaunchKernelGGL(k_y, grid, dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
+                    hipStream_t stream) {
+  ZParams P;
+  P.spec = spec;
+  P.kspec = kspec;
+  P.scale = scale;
+  P.hx = sp.hx;
+  P.qy = sp.q.y;
+  P.qz = sp.q.z;
+  P.zlo = zlo;
+  P.zhi = zhi;
+  P.lines = sp.bz;
+  P.plan = sp.pz;
+  P.tw = sp.tw_z;
+  dim3 grid((sp.hx + P.lines - 1) / P.lines, sp.q.y);
+  size_t lds = sizeof(float2) * (sp.q.z + (size_t)sp.q.z * (P.lines + 1));
+  hipLaunchKernelGGL(k_z, grid, dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+int x_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g) {
+  const int nrows = g.eff_size.y * g.eff_size.z;
+  return (nrows + sp.bx - 1) / sp.bx;
+}
+
+hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                            hipStream_t stream, int* nwg) {
+  XInvParams P;
+  P.g = g;
+  P.spec = spec;
+  P.e = e;
+  P.plan = sp.px;
+  P.tw = sp.tw_x;
+  P.hx = sp.hx;
+  P.lines = sp.bx;
+  const int M = g.q.x / 2;
+  const int n = x_inverse_workgroups(sp, g);
+  if (nwg) *nwg = n;
+  if (n == 0) return hipSuccess;
+  size_t lds = sizeof(float2) * (g.q.x + (size_t)(M + 1) * (P.lines + 1));
+  hipLaunchKernelGGL(k_x_inverse, dim3(n), dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+}  // namespace mvd

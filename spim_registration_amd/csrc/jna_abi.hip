@@ -1,0 +1,299 @@
+// The JNA ABI of the reference's native FFT-convolution library
+// (spim/process/cuda/CUDAStandardFunctions.java:26-45,
+//  spim/process/cuda/CUDAFourierConvolution.java:25-32), implemented on HIP.
+//
+// convolution3DfftCUDAInPlace is called by one Java thread per selected device
+// (deconv/MVDeconFFT.java:447-468), so every device has its own context: a
+// mutex, a stream, device buffers and a cache of kernel spectra keyed by the
+// kernel's bytes and the FFT size (the reference passes the same kernel for
+// every block of a view).  Nothing here may abort the JVM: errors are printed
+// to stderr and the block is left unchanged.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mvdecon.h"
+#include "mvd_internal.h"
+
+namespace mvd {
+namespace {
+
+uint64_t fnv1a(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < bytes; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+struct KernelSpectrum {
+  uint64_t hash;
+  Int3 kd, q;
+  float2* spec = nullptr;
+  size_t elems = 0;
+};
+
+struct DevicePlan {
+  Int3 q;
+  ShapePlan sp;
+};
+
+struct DeviceContext {
+  std::mutex mu;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  float* block = nullptr;
+  size_t block_cap = 0;
+  float* kern = nullptr;
+  size_t kern_cap = 0;
+  float2* spec = nullptr;
+  size_t spec_cap = 0;
+  std::list<DevicePlan> plans;
+  std::list<KernelSpectrum> kcache;  // most recent first
+};
+
+std::mutex g_ctx_mu;
+std::vector<std::unique_ptr<DeviceContext>> g_ctx;
+
+DeviceContext* context(int dev) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if ((int)g_ctx.size() <= dev) g_ctx.resize(dev + 1);
+  if (!g_ctx[dev]) {
+    g_ctx[dev] = std::make_unique<DeviceContext>();
+    g_ctx[dev]->device = dev;
+  }
+  return g_ctx[dev].get();
+}
+
+std::vector<float2> twiddle_table(int n) {
+  std::vector<float2> t(n);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = -two_pi * (long double)k / (long double)n;
+    t[k] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  return t;
+}
+
+int lines(int n) {
+  int b = 16;
+  while (b > 1 && n * b > kLdsCap) b >>= 1;
+  return b;
+}
+
+const ShapePlan* plan_for(DeviceContext* c, Int3 q) {
+  for (auto& p : c->plans)
+    if (p.q.x == q.x && p.q.y == q.y && p.q.z == q.z) return &p.sp;
+  if (q.x % 2 || !fft_size_supported(q.x / 2) || !fft_size_supported(q.y) || !fft_size_supported(q.z)) return nullptr;
+  DevicePlan dp;
+  dp.q = q;
+  ShapePlan& sp = dp.sp;
+  sp.q = q;
+  sp.hx = q.x / 2 + 1;
+  sp.px = make_plan(q.x / 2);
+  sp.py = make_plan(q.y);
+  sp.pz = make_plan(q.z);
+  sp.bx = lines(q.x / 2 + 1);
+  sp.by = lines(q.y);
+  sp.bz = lines(q.z);
+  float2** dst[3] = {&sp.tw_x, &sp.tw_y, &sp.tw_z};
+  const int len[3] = {q.x, q.y, q.z};
+  for (int i = 0; i < 3; ++i) {
+    std::vector<float2> t = twiddle_table(len[i]);
+    if (hipMalloc(dst[i], sizeof(float2) * len[i]) != hipSuccess) return nullptr;
+    if (hipMemcpy(*dst[i], t.data(), sizeof(float2) * len[i], hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  }
+  c->plans.push_back(dp);
+  return &c->plans.back().sp;
+}
+
+bool grow(void** p, size_t* cap, size_t bytes) {
+  if (bytes <= *cap) return true;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, bytes) != hipSuccess) return false;
+  *cap = bytes;
+  return true;
+}
+
+#define JNA_FAIL(msg)                                                    \
+  do {                                                                   \
+    std::fprintf(stderr, "libmvdecon_hip: %s\n", std::string(msg).c_str()); \
+    set_error(msg);                                                      \
+    return false;                                                        \
+  } while (0)
+
+#define JNA_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) JNA_FAIL(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Convolve the block `im` (dims x-fastest) with `kernel`; result into `out`.
+bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int dev, float* out) {
+  if (!im || !kernel || !out) JNA_FAIL("NULL array");
+  if (bd.x <= 0 || bd.y <= 0 || bd.z <= 0 || kd.x <= 0 || kd.y <= 0 || kd.z <= 0) JNA_FAIL("invalid dimensions");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || dev < 0 || dev >= ndev) JNA_FAIL("invalid device id");
+  DeviceContext* c = context(dev);
+  std::lock_guard<std::mutex> lk(c->mu);
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  struct Restore {
+    int d;
+    ~Restore() {
+      if (d >= 0) (void)hipSetDevice(d);
+    }
+  } restore{prev};
+  JNA_HIP(hipSetDevice(dev));
+  if (!c->stream) JNA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const Int3 q = {next_fft_size(bd.x, true), next_fft_size(bd.y, false), next_fft_size(bd.z, false)};
+  if (kd.x > q.x || kd.y > q.y || kd.z > q.z) JNA_FAIL("kernel larger than the block");
+  const ShapePlan* sp = plan_for(c, q);
+  if (!sp) JNA_FAIL("unsupported block size");
+  const size_t nb = (size_t)bd.x * bd.y * bd.z, nk = (size_t)kd.x * kd.y * kd.z;
+  if (!grow((void**)&c->block, &c->block_cap, sizeof(float) * nb)) JNA_FAIL("out of device memory (block)");
+  if (!grow((void**)&c->spec, &c->spec_cap, sizeof(float2) * sp->spectrum_elems()))
+    JNA_FAIL("out of device memory (spectrum)");
+  // kernel spectrum, cached by content
+  const uint64_t h = fnv1a(kernel, sizeof(float) * nk, fnv1a(&kd, sizeof(kd)));
+  KernelSpectrum* ks = nullptr;
+  for (auto it = c->kcache.begin(); it != c->kcache.end(); ++it)
+    if (it->hash == h && it->q.x == q.x && it->q.y == q.y && it->q.z == q.z && it->kd.x == kd.x &&
+        it->kd.y == kd.y && it->kd.z == kd.z) {
+      c->kcache.splice(c->kcache.begin(), c->kcache, it);
+      ks = &c->kcache.front();
+      break;
+    }
+  if (!ks) {
+    while (c->kcache.size() >= 16) {
+      (void)hipFree(c->kcache.back().spec);
+      c->kcache.pop_back();
+    }
+    KernelSpectrum k;
+    k.hash = h;
+    k.kd = kd;
+    k.q = q;
+    k.elems = sp->spectrum_elems();
+    JNA_HIP(hipMalloc(&k.spec, sizeof(float2) * k.elems));
+    c->kcache.push_front(k);
+    ks = &c->kcache.front();
+    if (!grow((void**)&c->kern, &c->kern_cap, sizeof(float) * nk)) JNA_FAIL("out of device memory (kernel)");
+    JNA_HIP(hipMemcpyAsync(c->kern, kernel, sizeof(float) * nk, hipMemcpyHostToDevice, c->stream));
+    ConvGeom g{};
+    g.src = kd;
+    g.win = q;
+    g.q = q;
+    JNA_HIP(launch_x_forward(*sp, g, c->kern, kGatherKernelWrap, ks->spec, c->stream));
+    JNA_HIP(launch_y(*sp, ks->spec, 0, q.z, 0, q.y, -1.f, c->stream));
+    const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
+    JNA_HIP(launch_z(*sp, ks->spec, nullptr, scale, 0, q.z, c->stream));
+  }
+  JNA_HIP(hipMemcpyAsync(c->block, im, sizeof(float) * nb, hipMemcpyHostToDevice, c->stream));
+  ConvGeom g{};
+  g.src = bd;
+  g.off = {0, 0, 0};
+  g.win = bd;
+  g.q = q;
+  g.eff_local = {0, 0, 0};
+  g.eff_size = bd;
+  g.eff_offset = {0, 0, 0};
+  g.dst = bd;
+  EpilogueArgs e{};
+  e.mode = kEpiStore;
+  e.dst = c->block;
+  JNA_HIP(launch_x_forward(*sp, g, c->block, kGatherZero, c->spec, c->stream));
+  JNA_HIP(launch_y(*sp, c->spec, 0, q.z, 0, q.y, -1.f, c->stream));
+  JNA_HIP(launch_z(*sp, c->spec, ks->spec, 1.f, 0, bd.z, c->stream));
+  JNA_HIP(launch_y(*sp, c->spec, 0, bd.z, 0, bd.y, 1.f, c->stream));
+  JNA_HIP(launch_x_inverse(*sp, g, c->spec, e, c->stream, nullptr));
+  JNA_HIP(hipMemcpyAsync(out, c->block, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream));
+  JNA_HIP(hipStreamSynchronize(c->stream));
+  return true;
+}
+
+bool dims_ok(const int* d) { return d && d[0] > 0 && d[1] > 0 && d[2] > 0; }
+
+}  // namespace
+}  // namespace mvd
+
+using namespace mvd;
+
+extern "C" {
+
+int getNumDevicesCUDA(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+  return n;
+}
+
+void getNameDeviceCUDA(int devCUDA, char* name) {
+  if (!name) return;
+  name[0] = 0;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return;
+  std::strncpy(name, p.name, 255);
+  name[255] = 0;
+}
+
+long long getMemDeviceCUDA(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return (long long)p.totalGlobalMem;
+}
+
+long long getFreeMemDeviceCUDA(int devCUDA) {
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  if (hipSetDevice(devCUDA) != hipSuccess) return -1;
+  size_t fr = 0, tot = 0;
+  hipError_t e = hipMemGetInfo(&fr, &tot);
+  if (prev >= 0) (void)hipSetDevice(prev);
+  return e == hipSuccess ? (long long)fr : -1;
+}
+
+int getCUDAcomputeCapabilityMajorVersion(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return p.major;
+}
+
+int getCUDAcomputeCapabilityMinorVersion(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return p.minor;
+}
+
+void convolution3DfftCUDAInPlace(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA) {
+  if (!dims_ok(imDim) || !dims_ok(kernelDim)) {
+    std::fprintf(stderr, "libmvdecon_hip: invalid dimensions\n");
+    return;
+  }
+  // dims arrive reversed: {nz, ny, nx} (MVDeconFFTThreads.getCUDACoordinates, :157-165)
+  const Int3 bd = {imDim[2], imDim[1], imDim[0]}, kd = {kernelDim[2], kernelDim[1], kernelDim[0]};
+  (void)convolve_block(im, bd, kernel, kd, devCUDA, im);
+}
+
+float* convolution3DfftCUDA(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA) {
+  if (!dims_ok(imDim) || !dims_ok(kernelDim)) return nullptr;
+  const Int3 bd = {imDim[2], imDim[1], imDim[0]}, kd = {kernelDim[2], kernelDim[1], kernelDim[0]};
+  const size_t nb = (size_t)bd.x * bd.y * bd.z;
+  float* out = static_cast<float*>(std::malloc(sizeof(float) * nb));
+  if (!out) return nullptr;
+  if (!convolve_block(im, bd, kernel, kd, devCUDA, out)) {
+    std::free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+void mvd_free(void* p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Internal declarations shared by the HIP translation units of libmvdecon_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fft_device.h"
+
+namespace mvd {
+
+// How the R2C pass fills the FFT window from its source volume.
+enum Gather : int {
+  kGatherMirror = 0,      // mirror-single out-of-bounds (psi, MVDeconFFTThreads.java:44,78)
+  kGatherOne = 1,         // constant 1 out of bounds (ratio, MVDeconFFT.java:515)
+  kGatherZero = 2,        // zero out of bounds (compound kernels, MVDeconFFT.java:222-228)
+  kGatherKernelWrap = 3,  // kernel wrapped so that its centre k/2 sits at index 0
+};
+
+// What the C2R pass does with the convolution result.
+enum Epilogue : int {
+  kEpiStore = 0,     // dst = result
+  kEpiQuotient = 1,  // dst = img > 0 ? img / result : 1   (MVDeconvolution.java:494-546)
+  kEpiUpdate = 2,    // psi update + statistics          (MVDeconvolution.java:603-724)
+};
+
+// Weight source used by the update epilogue.
+enum WeightMode : int {
+  kWeightNone = 0,     // w = 1 (ProcessForDeconvolution.java:223-224)
+  kWeightArray = 1,    // w = weight[i] (precomputed, already normalised)
+  kWeightVirtual = 2,  // w = min(1, weight[i] / sum[i] * osem) in double (NormalizingRandomAccess.java:56-66)
+};
+
+struct Int3 {
+  int x, y, z;
+};
+
+// Geometry of one FFT convolution job: a window of the source volume is
+// convolved with circular wrap over the padded size q, and the sub-box
+// [eff_local, eff_local + eff_size) of the window is written to the
+// destination at eff_offset.
+struct ConvGeom {
+  Int3 src;         // source volume size
+  Int3 off;         // window origin in source coordinates (may be negative)
+  Int3 win;         // window size (data); [win, q) is zero padding
+  Int3 q;           // FFT size (q.x even)
+  Int3 eff_local;   // first output voxel inside the window
+  Int3 eff_size;    // output box size
+  Int3 eff_offset;  // output box origin in destination coordinates
+  Int3 dst;         // destination volume size
+};
+
+// Device-resident plan of one FFT size: twiddle tables and radix plans.
+struct ShapePlan {
+  Int3 q;
+  int hx;                    // q.x / 2 + 1 (half-spectrum row length)
+  FftPlan px, py, pz;        // plans for lengths q.x/2, q.y, q.z
+  float2* tw_x = nullptr;    // q.x entries: exp(-2 pi i m / q.x)
+  float2* tw_y = nullptr;    // q.y entries
+  float2* tw_z = nullptr;    // q.z entries
+  int bx, by, bz;            // lines per workgroup for the x, y, z passes
+  int64_t spectrum_elems() const { return (int64_t)hx * q.y * q.z; }
+};
+
+FftPlan make_plan(int n);
+bool fft_size_supported(int n);
+int next_fft_size(int n, bool even);
+
+struct EpilogueArgs {
+  int mode;              // Epilogue
+  float* dst;            // store/quotient target, or psi for the update
+  const float* img;      // observed view for the quotient
+  const float* weight;   // weights for the update (WeightMode)
+  const float* wsum;     // sum image for virtual weights
+  int weight_mode;
+  double osem;
+  double lambda;
+  double* part_sum;      // per-workgroup statistics of the update
+  float* part_max;
+};
+
+// Launchers (all asynchronous on `stream`).
+hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                            hipStream_t stream);
+hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                    hipStream_t stream);
+// kspec == nullptr: forward z transform then multiply by `scale` (kernel spectra).
+hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
+                    hipStream_t stream);
+// Returns the number of workgroups (for the update statistics) in *nwg.
+hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                            hipStream_t stream, int* nwg);
+int x_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g);
+
+hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, int n, double* out,
+                               hipStream_t stream);
+hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
+                                  unsigned long long* part_cnt, int nparts, hipStream_t stream);
+hipError_t launch_fill(float* dst, float v, int64_t n, hipStream_t stream);
+hipError_t launch_mask(const float* psi, const float* counts, float* out, int64_t n, hipStream_t stream);
+hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_t stride, float* wsum,
+                                   int virtual_mode, int* part_min, unsigned long long* part_cnt, int nparts,
+                                   hipStream_t stream);
+hipError_t launch_osem_scale(float* weights, int64_t n, float osem, hipStream_t stream);
+
+// Plan cache and kernel-spectrum helpers (session.cpp).
+void set_error(const std::string& msg);
+
+}  // namespace mvd

@@ -1,0 +1,945 @@
+// Deconvolution session: state resident in HBM and the host-side driver of
+// the MVDeconvolution iteration (reference: deconv/MVDeconvolution.java,
+// deconv/MVDeconFFT.java, deconv/MVDeconInput.java, cuda/Block*.java).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mvdecon.h"
+#include "mvd_internal.h"
+
+namespace mvd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+#define MVD_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+      return -2;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define MVD_REQUIRE(cond, msg) \
+  do {                         \
+    if (!(cond)) {             \
+      set_error(msg);          \
+      return -1;               \
+    }                          \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// FFT plans
+
+static bool is_5smooth(int n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// Lengths up to 1024 keep every pass within 64 KB of LDS per workgroup.
+bool fft_size_supported(int n) { return n >= 1 && n <= 1024 && is_5smooth(n); }
+
+int next_fft_size(int n, bool even) {
+  int m = std::max(n, even ? 2 : 1);
+  for (;; ++m)
+    if (is_5smooth(m) && (!even || m % 2 == 0)) return m;
+}
+
+FftPlan make_plan(int n) {
+  FftPlan p{};
+  p.n = n;
+  int m = n, ns = 1;
+  auto push = [&](int r) {
+    p.radix[p.npass] = r;
+    p.ns[p.npass] = ns;
+    ns *= r;
+    ++p.npass;
+    m /= r;
+  };
+  while (m % 8 == 0) push(8);
+  while (m % 4 == 0) push(4);
+  while (m % 2 == 0) push(2);
+  while (m % 3 == 0) push(3);
+  while (m % 5 == 0) push(5);
+  return p;
+}
+
+static int lines_for(int n, int want) {
+  int b = want;
+  while (b > 1 && n * b > kLdsCap) b >>= 1;
+  return b;
+}
+
+static std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = -two_pi * (long double)k / (long double)n;
+    t[k] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  return t;
+}
+
+struct DeviceBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DeviceBuf() = default;
+  DeviceBuf(const DeviceBuf&) = delete;
+  DeviceBuf& operator=(const DeviceBuf&) = delete;
+  DeviceBuf(DeviceBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DeviceBuf& operator=(DeviceBuf&& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+    return *this;
+  }
+  ~DeviceBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+class PlanCache {
+ public:
+  ~PlanCache() {
+    for (auto& kv : plans_) {
+      (void)hipFree(kv.second->tw_x);
+      (void)hipFree(kv.second->tw_y);
+      (void)hipFree(kv.second->tw_z);
+    }
+  }
+  // Returns nullptr (and sets the error) for unsupported sizes.
+  const ShapePlan* get(Int3 q) {
+    auto key = std::make_tuple(q.x, q.y, q.z);
+    auto it = plans_.find(key);
+    if (it != plans_.end()) return it->second.get();
+    if (q.x % 2 || !fft_size_supported(q.x / 2) || !fft_size_supported(q.y) || !fft_size_supported(q.z)) {
+      set_error("unsupported FFT size " + std::to_string(q.x) + "x" + std::to_string(q.y) + "x" +
+                std::to_string(q.z));
+      return nullptr;
+    }
+    auto sp = std::make_unique<ShapePlan>();
+    sp->q = q;
+    sp->hx = q.x / 2 + 1;
+    sp->px = make_plan(q.x / 2);
+    sp->py = make_plan(q.y);
+    sp->pz = make_plan(q.z);
+    sp->bx = lines_for(q.x / 2 + 1, 16);
+    sp->by = lines_for(q.y, 16);
+    sp->bz = lines_for(q.z, 16);
+    auto up = [](int n, float2** dst) -> bool {
+      std::vector<float2> t = twiddles(n);
+      if (hipMalloc(dst, sizeof(float2) * n) != hipSuccess) return false;
+      return hipMemcpy(*dst, t.data(), sizeof(float2) * n, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(q.x, &sp->tw_x) || !up(q.y, &sp->tw_y) || !up(q.z, &sp->tw_z)) {
+      set_error("twiddle table allocation failed");
+      return nullptr;
+    }
+    const ShapePlan* r = sp.get();
+    plans_[key] = std::move(sp);
+    return r;
+  }
+
+ private:
+  std::map<std::tuple<int, int, int>, std::unique_ptr<ShapePlan>> plans_;
+};
+
+// ---------------------------------------------------------------------------
+// Timing of kernel classes with HIP events on the session stream.
+
+enum TimingClass {
+  kTXForward = 0,
+  kTYForward,
+  kTZConv,
+  kTYInverse,
+  kTXInvQuotient,
+  kTXInvUpdate,
+  kTXInvStore,
+  kTZForward,
+  kTReduce,
+  kTPointwise,
+  kTNumClasses
+};
+static const char* kTimingNames[kTNumClasses] = {"x_forward",         "y_forward",       "z_conv",
+                                                 "y_inverse",         "x_inverse_quotient", "x_inverse_update",
+                                                 "x_inverse_store",   "z_forward",       "reduce_stats",
+                                                 "pointwise"};
+
+class Timer {
+ public:
+  ~Timer() { release(); }
+  void enable(bool on) {
+    on_ = on;
+    std::fill(ms_, ms_ + kTNumClasses, 0.0);
+    std::fill(cnt_, cnt_ + kTNumClasses, 0);
+  }
+  bool on() const { return on_; }
+  void begin(hipStream_t s) {
+    if (!on_) return;
+    if (used_ == pool_.size()) {
+      hipEvent_t a, b;
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      pool_.push_back({a, b, 0});
+    }
+    (void)hipEventRecord(pool_[used_].start, s);
+  }
+  void end(hipStream_t s, int cls) {
+    if (!on_) return;
+    (void)hipEventRecord(pool_[used_].stop, s);
+    pool_[used_].cls = cls;
+    ++used_;
+  }
+  // Call after the stream is synchronised.
+  void collect() {
+    for (size_t i = 0; i < used_; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pool_[i].start, pool_[i].stop) == hipSuccess) {
+        ms_[pool_[i].cls] += ms;
+        cnt_[pool_[i].cls] += 1;
+      }
+    }
+    used_ = 0;
+  }
+  double ms(int c) const { return ms_[c]; }
+  long long count(int c) const { return cnt_[c]; }
+
+ private:
+  struct Pair {
+    hipEvent_t start, stop;
+    int cls;
+  };
+  void release() {
+    for (auto& p : pool_) {
+      (void)hipEventDestroy(p.start);
+      (void)hipEventDestroy(p.stop);
+    }
+    pool_.clear();
+  }
+  bool on_ = false;
+  std::vector<Pair> pool_;
+  size_t used_ = 0;
+  double ms_[kTNumClasses] = {};
+  long long cnt_[kTNumClasses] = {};
+};
+
+// ---------------------------------------------------------------------------
+// Block geometry: BlockGeneratorFixedSizePrecise.divideIntoBlocks
+// (cuda/BlockGeneratorFixedSizePrecise.java:46-122).  Effective size bs-k+1,
+// effective local offset k/2, blocks x-fastest, last block clipped.
+
+static bool divide_into_blocks(Int3 img, Int3 ker, Int3 bs, Int3 q, std::vector<ConvGeom>* out) {
+  const int is[3] = {img.x, img.y, img.z}, ks[3] = {ker.x, ker.y, ker.z}, b[3] = {bs.x, bs.y, bs.z};
+  int eff[3], nb[3];
+  for (int d = 0; d < 3; ++d) {
+    eff[d] = b[d] - ks[d] + 1;
+    if (eff[d] <= 0) return false;
+    nb[d] = is[d] / eff[d] + (is[d] % eff[d] != 0);
+  }
+  out->clear();
+  for (int k = 0; k < nb[2]; ++k)
+    for (int j = 0; j < nb[1]; ++j)
+      for (int i = 0; i < nb[0]; ++i) {
+        const int cur[3] = {i, j, k};
+        int off[3], esz[3], eoff[3], eloc[3];
+        for (int d = 0; d < 3; ++d) {
+          esz[d] = eff[d];
+          eoff[d] = cur[d] * eff[d];
+          off[d] = eoff[d] - ks[d] / 2;
+          eloc[d] = ks[d] / 2;
+          if (eoff[d] + esz[d] > is[d]) esz[d] = is[d] - eoff[d];
+        }
+        ConvGeom g;
+        g.src = img;
+        g.off = {off[0], off[1], off[2]};
+        g.win = bs;
+        g.q = q;
+        g.eff_local = {eloc[0], eloc[1], eloc[2]};
+        g.eff_size = {esz[0], esz[1], esz[2]};
+        g.eff_offset = {eoff[0], eoff[1], eoff[2]};
+        g.dst = img;
+        out->push_back(g);
+      }
+  return true;
+}
+
+static Int3 fft_shape_for(Int3 win) {
+  return {next_fft_size(win.x, true), next_fft_size(win.y, false), next_fft_size(win.z, false)};
+}
+
+// ---------------------------------------------------------------------------
+// Host-side kernel preparation helpers (tiny arrays).
+
+// FusionHelper.divideIntoPortions (spim/process/fusion/FusionHelper.java:257-280).
+static std::vector<std::pair<int64_t, int64_t>> portions(int64_t n, int k) {
+  std::vector<std::pair<int64_t, int64_t>> p;
+  const int64_t chunk = n / k, mod = n % k;
+  for (int i = 0; i < k; ++i) p.push_back({i * chunk, i == k - 1 ? chunk + mod : chunk});
+  return p;
+}
+
+// Pairwise summation of doubles in input order (binary counter).
+class PairSum {
+ public:
+  void add(double a) {
+    size_t i = 0;
+    double s = a;
+    while (i < full_.size() && full_[i]) {
+      full_[i] = false;
+      s += v_[i];
+      v_[i] = 0.0;
+      ++i;
+    }
+    if (i == full_.size()) {
+      full_.push_back(false);
+      v_.push_back(0.0);
+    }
+    full_[i] = true;
+    v_[i] = s;
+  }
+  double sum() const {
+    double s = 0;
+    for (double x : v_) s += x;
+    return s;
+  }
+
+ private:
+  std::vector<bool> full_;
+  std::vector<double> v_;
+};
+
+// AdjustInput.normImg (deconv/AdjustInput.java:50-100): k /= S where S sums
+// 2*threads portions and counts the first portion twice.
+static void norm_img(std::vector<float>& k, int threads) {
+  auto ps = portions((int64_t)k.size(), 2 * threads);
+  std::vector<double> part;
+  for (auto& p : ps) {
+    PairSum s;
+    for (int64_t j = 0; j < p.second; ++j) s.add((double)k[p.first + j]);
+    part.push_back(s.sum());
+  }
+  PairSum total;
+  total.add(part[0]);
+  for (double v : part) total.add(v);
+  const double sum = total.sum();
+  for (auto& v : k) v = (float)((double)v / sum);
+}
+
+// Mirror.mirror along each axis (deconv/Mirror.java:52-129; MVDeconFFT.computeInvertedKernel,
+// deconv/MVDeconFFT.java:336-344).  Positions p <= n/2 are swapped with n-1-p, so
+// for even n the middle pair is swapped twice and stays.
+static std::vector<float> inverted(const std::vector<float>& k, Int3 d) {
+  std::vector<float> r = k;
+  const int dims[3] = {d.x, d.y, d.z};
+  for (int axis = 0; axis < 3; ++axis) {
+    const int len = dims[axis], half = len / 2, mx = len - 1;
+    for (int z = 0; z < d.z; ++z)
+      for (int y = 0; y < d.y; ++y)
+        for (int x = 0; x < d.x; ++x) {
+          int p[3] = {x, y, z};
+          if (p[axis] > half) continue;
+          p[axis] = mx - p[axis];
+          std::swap(r[x + (size_t)d.x * (y + (size_t)d.y * z)], r[p[0] + (size_t)d.x * (p[1] + (size_t)d.y * p[2])]);
+        }
+  }
+  return r;
+}
+
+}  // namespace mvd
+
+using namespace mvd;
+
+struct mvd_session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Int3 img{0, 0, 0};
+  int64_t n = 0;
+  int nviews = 0;
+
+  struct View {
+    std::vector<float> psf_raw;  // as given
+    std::vector<float> k1, k2;   // kernel1 (normalised) and kernel2
+    Int3 kd{0, 0, 0};
+    bool has_weight = false;
+    const ShapePlan* plan = nullptr;
+    std::vector<ConvGeom> blocks;
+    DeviceBuf k1hat, k2hat;
+    int nwg_update = 0;
+  };
+  std::vector<View> views;
+
+  DeviceBuf d_views, d_weights, d_wsum, d_psi, d_ratio, d_spec, d_part_sum, d_part_max, d_stats, d_tmp;
+  DeviceBuf d_fi_sum, d_fi_cnt;
+  PlanCache plans;
+  Timer timer;
+
+  int psf_type = MVD_PSF_EFFICIENT_BAYESIAN;
+  double lambda = 0.0;
+  int norm_threads = 8;
+  bool use_blocks = false;
+  Int3 block{0, 0, 0};
+  int weight_mode = -1;  // -1: decided at prepare
+  double osem = 1.0;
+  bool prepared = false, psi_ready = false;
+
+  ~mvd_session() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// One FFT convolution of a window: forward passes, kernel product, inverse
+// passes and the fused epilogue, all on s->stream.
+int run_conv(mvd_session* s, const ShapePlan& sp, const ConvGeom& g, const float* src, int gather,
+             const float2* khat, const EpilogueArgs& e, float2* spec) {
+  hipStream_t st = s->stream;
+  Timer& T = s->timer;
+  const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
+  const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
+  T.begin(st);
+  MVD_CHECK(launch_x_forward(sp, g, src, gather, spec, st));
+  T.end(st, kTXForward);
+  T.begin(st);
+  MVD_CHECK(launch_y(sp, spec, 0, sp.q.z, 0, sp.q.y, -1.f, st));
+  T.end(st, kTYForward);
+  T.begin(st);
+  MVD_CHECK(launch_z(sp, spec, khat, 1.f, zlo, zhi, st));
+  T.end(st, kTZConv);
+  T.begin(st);
+  MVD_CHECK(launch_y(sp, spec, zlo, zhi, ylo, yhi, 1.f, st));
+  T.end(st, kTYInverse);
+  T.begin(st);
+  int nwg = 0;
+  MVD_CHECK(launch_x_inverse(sp, g, spec, e, st, &nwg));
+  T.end(st, e.mode == kEpiQuotient ? kTXInvQuotient : e.mode == kEpiUpdate ? kTXInvUpdate : kTXInvStore);
+  return 0;
+}
+
+// Spectrum of a kernel wrapped into the FFT box of sp, scaled by 1/(qx qy qz).
+int kernel_spectrum(mvd_session* s, const ShapePlan& sp, const std::vector<float>& k, Int3 kd, DeviceBuf* out) {
+  MVD_CHECK(out->ensure(sizeof(float2) * sp.spectrum_elems()));
+  DeviceBuf dk;
+  MVD_CHECK(dk.ensure(sizeof(float) * k.size()));
+  MVD_CHECK(hipMemcpyAsync(dk.p, k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s->stream));
+  ConvGeom g{};
+  g.src = kd;
+  g.off = {0, 0, 0};
+  g.win = sp.q;
+  g.q = sp.q;
+  Timer& T = s->timer;
+  T.begin(s->stream);
+  MVD_CHECK(launch_x_forward(sp, g, dk.as<float>(), kGatherKernelWrap, out->as<float2>(), s->stream));
+  T.end(s->stream, kTXForward);
+  T.begin(s->stream);
+  MVD_CHECK(launch_y(sp, out->as<float2>(), 0, sp.q.z, 0, sp.q.y, -1.f, s->stream));
+  T.end(s->stream, kTYForward);
+  const float scale = (float)(1.0L / ((long double)sp.q.x * sp.q.y * sp.q.z));
+  T.begin(s->stream);
+  MVD_CHECK(launch_z(sp, out->as<float2>(), nullptr, scale, 0, sp.q.z, s->stream));
+  T.end(s->stream, kTZForward);
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+// "Same"-size convolution with zero out-of-bounds, as the compound-kernel
+// construction calls FFTConvolution (deconv/MVDeconFFT.java:222-248, 285-295).
+int same_conv_zero(mvd_session* s, const std::vector<float>& in, Int3 din, const std::vector<float>& ker, Int3 dk,
+                   std::vector<float>* out) {
+  ConvGeom g{};
+  g.src = din;
+  g.off = {-dk.x / 2, -dk.y / 2, -dk.z / 2};
+  g.win = {din.x + dk.x - 1, din.y + dk.y - 1, din.z + dk.z - 1};
+  g.q = fft_shape_for(g.win);
+  g.eff_local = {dk.x / 2, dk.y / 2, dk.z / 2};
+  g.eff_size = din;
+  g.eff_offset = {0, 0, 0};
+  g.dst = din;
+  const ShapePlan* sp = s->plans.get(g.q);
+  if (!sp) return -1;
+  DeviceBuf khat, spec, din_buf, dout;
+  int rc = kernel_spectrum(s, *sp, ker, dk, &khat);
+  if (rc) return rc;
+  MVD_CHECK(spec.ensure(sizeof(float2) * sp->spectrum_elems()));
+  MVD_CHECK(din_buf.ensure(sizeof(float) * in.size()));
+  MVD_CHECK(dout.ensure(sizeof(float) * in.size()));
+  MVD_CHECK(hipMemcpyAsync(din_buf.p, in.data(), sizeof(float) * in.size(), hipMemcpyHostToDevice, s->stream));
+  EpilogueArgs e{};
+  e.mode = kEpiStore;
+  e.dst = dout.as<float>();
+  rc = run_conv(s, *sp, g, din_buf.as<float>(), kGatherZero, khat.as<float2>(), e, spec.as<float2>());
+  if (rc) return rc;
+  out->resize(in.size());
+  MVD_CHECK(hipMemcpyAsync(out->data(), dout.p, sizeof(float) * in.size(), hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+// MVDeconFFT.init for one view (deconv/MVDeconFFT.java:183-324).
+int init_view(mvd_session* s, int v) {
+  auto& V = s->views;
+  mvd_session::View& view = V[v];
+  norm_img(view.k1, s->norm_threads);
+  const int nv = s->nviews;
+  const int64_t kn = (int64_t)view.k1.size();
+  if (nv == 1 || s->psf_type == MVD_PSF_INDEPENDENT) {
+    view.k2 = inverted(view.k1, view.kd);
+  } else if (s->psf_type == MVD_PSF_EFFICIENT_BAYESIAN) {
+    std::vector<float> tmp = inverted(view.k1, view.kd);
+    for (int w = 0; w < nv; ++w) {
+      if (w == v) continue;
+      std::vector<float> a, b;
+      int rc = same_conv_zero(s, inverted(view.k1, view.kd), view.kd, V[w].k1, V[w].kd, &a);
+      if (rc) return rc;
+      rc = same_conv_zero(s, a, view.kd, inverted(V[w].k1, V[w].kd), V[w].kd, &b);
+      if (rc) return rc;
+      for (int64_t i = 0; i < kn; ++i) tmp[i] = b[i] * tmp[i];
+    }
+    norm_img(tmp, s->norm_threads);
+    view.k2 = tmp;
+  } else if (s->psf_type == MVD_PSF_OPTIMIZATION_I) {
+    std::vector<float> tmp = view.k1;
+    for (int w = 0; w < nv; ++w) {
+      if (w == v) continue;
+      std::vector<float> a;
+      int rc = same_conv_zero(s, view.k1, view.kd, inverted(V[w].k1, V[w].kd), V[w].kd, &a);
+      if (rc) return rc;
+      for (int64_t i = 0; i < kn; ++i) tmp[i] = a[i] * tmp[i];
+    }
+    norm_img(tmp, s->norm_threads);
+    view.k2 = inverted(tmp, view.kd);
+  } else {
+    // OPTIMIZATION_II: float power by repeated product (MVDeconFFT.java:326-354)
+    std::vector<float> e = view.k1;
+    for (auto& f : e) {
+      float r = f;
+      for (int i = 1; i < nv; ++i) r *= f;
+      f = r;
+    }
+    norm_img(e, s->norm_threads);
+    view.k2 = inverted(e, view.kd);
+  }
+  return 0;
+}
+
+int build_geometry(mvd_session* s, int v) {
+  mvd_session::View& view = s->views[v];
+  Int3 bs = s->use_blocks ? s->block
+                          : Int3{s->img.x + view.kd.x - 1, s->img.y + view.kd.y - 1, s->img.z + view.kd.z - 1};
+  const Int3 q = fft_shape_for(bs);
+  view.plan = s->plans.get(q);
+  if (!view.plan) return -1;
+  MVD_REQUIRE(divide_into_blocks(s->img, view.kd, bs, q, &view.blocks),
+              "block size smaller than the kernel (BlockGeneratorFixedSizePrecise)");
+  view.nwg_update = 0;
+  for (auto& g : view.blocks) view.nwg_update += x_inverse_workgroups(*view.plan, g);
+  return 0;
+}
+
+int view_spectra(mvd_session* s, int v) {
+  mvd_session::View& view = s->views[v];
+  int rc = kernel_spectrum(s, *view.plan, view.k1, view.kd, &view.k1hat);
+  if (rc) return rc;
+  return kernel_spectrum(s, *view.plan, view.k2, view.kd, &view.k2hat);
+}
+
+int ensure_workspace(mvd_session* s) {
+  size_t spec = 0;
+  int parts = 1;
+  for (auto& view : s->views) {
+    spec = std::max<size_t>(spec, (size_t)view.plan->spectrum_elems());
+    parts = std::max(parts, view.nwg_update);
+  }
+  MVD_CHECK(s->d_spec.ensure(sizeof(float2) * spec));
+  MVD_CHECK(s->d_part_sum.ensure(sizeof(double) * parts));
+  MVD_CHECK(s->d_part_max.ensure(sizeof(float) * parts));
+  return 0;
+}
+
+constexpr int kReduceParts = 2048;
+
+// Sum of the per-voxel means and number of voxels with data (FirstIteration).
+int first_iteration_sums(mvd_session* s, float* counts, double* sum, unsigned long long* cnt) {
+  MVD_CHECK(s->d_fi_sum.ensure(sizeof(double) * kReduceParts));
+  MVD_CHECK(s->d_fi_cnt.ensure(sizeof(unsigned long long) * kReduceParts));
+  const int parts = (int)std::min<int64_t>(kReduceParts, std::max<int64_t>(1, (s->n + 255) / 256));
+  MVD_CHECK(launch_first_iteration(s->d_views.as<float>(), s->nviews, s->n, counts,
+                                   s->d_fi_sum.as<double>(), s->d_fi_cnt.as<unsigned long long>(), parts,
+                                   s->stream));
+  std::vector<double> ps(parts);
+  std::vector<unsigned long long> pc(parts);
+  MVD_CHECK(hipMemcpyAsync(ps.data(), s->d_fi_sum.p, sizeof(double) * parts, hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipMemcpyAsync(pc.data(), s->d_fi_cnt.p, sizeof(unsigned long long) * parts, hipMemcpyDeviceToHost,
+                           s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  double t = 0;
+  unsigned long long c = 0;
+  for (int i = 0; i < parts; ++i) {
+    t += ps[i];
+    c += pc[i];
+  }
+  *sum = t;
+  *cnt = c;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mvd_last_error(void) { return g_last_error.c_str(); }
+
+int mvd_create(int device, int nx, int ny, int nz, int nviews, mvd_session** out) {
+  MVD_REQUIRE(out != nullptr, "out is NULL");
+  *out = nullptr;
+  MVD_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nviews > 0, "invalid image size or view count");
+  int ndev = 0;
+  MVD_CHECK(hipGetDeviceCount(&ndev));
+  MVD_REQUIRE(device >= 0 && device < ndev, "invalid device id");
+  DeviceGuard guard(device);
+  auto s = std::make_unique<mvd_session>();
+  s->device = device;
+  s->img = {nx, ny, nz};
+  s->n = (int64_t)nx * ny * nz;
+  s->nviews = nviews;
+  s->views.reserve(nviews);
+  MVD_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  MVD_CHECK(s->d_views.ensure(sizeof(float) * s->n * nviews));
+  MVD_CHECK(s->d_psi.ensure(sizeof(float) * s->n));
+  MVD_CHECK(s->d_ratio.ensure(sizeof(float) * s->n));
+  *out = s.release();
+  return 0;
+}
+
+int mvd_destroy(mvd_session* s) {
+  if (!s) return 0;
+  DeviceGuard guard(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  delete s;
+  return 0;
+}
+
+int mvd_add_view(mvd_session* s, const float* img, const float* weight, const float* psf, int kx, int ky, int kz) {
+  MVD_REQUIRE(s && img && psf, "NULL argument");
+  MVD_REQUIRE((int)s->views.size() < s->nviews, "all views already added");
+  MVD_REQUIRE(kx > 0 && ky > 0 && kz > 0 && (kx & 1) && (ky & 1) && (kz & 1),
+              "kernel sizes must be odd (BlockGeneratorFixedSizePrecise)");
+  MVD_REQUIRE(kx <= s->img.x + 4096 && ky <= s->img.y + 4096, "kernel too large");
+  DeviceGuard guard(s->device);
+  const int v = (int)s->views.size();
+  if (weight) {
+    MVD_CHECK(s->d_weights.ensure(sizeof(float) * s->n * s->nviews));
+    MVD_CHECK(hipMemcpyAsync(s->d_weights.as<float>() + v * s->n, weight, sizeof(float) * s->n,
+                             hipMemcpyHostToDevice, s->stream));
+  }
+  MVD_CHECK(hipMemcpyAsync(s->d_views.as<float>() + v * s->n, img, sizeof(float) * s->n, hipMemcpyHostToDevice,
+                           s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  s->views.emplace_back();
+  auto& view = s->views.back();
+  view.kd = {kx, ky, kz};
+  view.psf_raw.assign(psf, psf + (size_t)kx * ky * kz);
+  view.has_weight = weight != nullptr;
+  s->prepared = false;
+  return 0;
+}
+
+int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value, int norm_threads,
+                    double* min_avg_out) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
+  MVD_REQUIRE(mode >= MVD_WEIGHTS_NONE && mode <= MVD_WEIGHTS_VIRTUAL, "invalid weight mode");
+  MVD_REQUIRE(norm_threads >= 1, "norm_threads must be >= 1");
+  bool all_w = true;
+  for (auto& v : s->views) all_w = all_w && v.has_weight;
+  MVD_REQUIRE(mode == MVD_WEIGHTS_NONE || all_w, "this weight mode needs a weight image for every view");
+  DeviceGuard guard(s->device);
+  double mn = 1, avg = 1;
+  if (mode == MVD_WEIGHTS_PRECOMPUTED || mode == MVD_WEIGHTS_VIRTUAL) {
+    // WeightNormalizer.process (deconv/WeightNormalizer.java:73-130): per portion
+    // of 2*threads, min count and average count; the overall average is the mean
+    // of the portion averages.
+    if (mode == MVD_WEIGHTS_VIRTUAL) MVD_CHECK(s->d_wsum.ensure(sizeof(float) * s->n));
+    DeviceBuf pmin, pcnt;
+    MVD_CHECK(pmin.ensure(sizeof(int) * kReduceParts));
+    MVD_CHECK(pcnt.ensure(sizeof(unsigned long long) * kReduceParts));
+    int gmin = s->nviews;
+    double avg_sum = 0;
+    auto ps = portions(s->n, 2 * norm_threads);
+    for (auto& p : ps) {
+      if (p.second == 0) {
+        avg_sum += std::nan("");
+        continue;
+      }
+      const int parts = (int)std::min<int64_t>(kReduceParts, std::max<int64_t>(1, (p.second + 255) / 256));
+      MVD_CHECK(launch_weight_normalize(s->d_weights.as<float>() + p.first, s->nviews, p.second, s->n,
+                                        mode == MVD_WEIGHTS_VIRTUAL ? s->d_wsum.as<float>() + p.first : nullptr,
+                                        mode == MVD_WEIGHTS_VIRTUAL, pmin.as<int>(),
+                                        pcnt.as<unsigned long long>(), parts, s->stream));
+      std::vector<int> hm(parts);
+      std::vector<unsigned long long> hc(parts);
+      MVD_CHECK(hipMemcpyAsync(hm.data(), pmin.p, sizeof(int) * parts, hipMemcpyDeviceToHost, s->stream));
+      MVD_CHECK(hipMemcpyAsync(hc.data(), pcnt.p, sizeof(unsigned long long) * parts, hipMemcpyDeviceToHost,
+                               s->stream));
+      MVD_CHECK(hipStreamSynchronize(s->stream));
+      unsigned long long c = 0;
+      for (int i = 0; i < parts; ++i) {
+        gmin = std::min(gmin, hm[i]);
+        c += hc[i];
+      }
+      avg_sum += (double)c / (double)p.second;
+    }
+    mn = std::max(1, gmin);
+    avg = std::max(1.0, avg_sum / ps.size());
+  }
+  double osem = 1.0;
+  if (osem_index == 1)
+    osem = mn;
+  else if (osem_index == 2)
+    osem = avg;
+  else if (osem_index == 3)
+    osem = osem_value;
+  // ProcessForDeconvolution.adjustForOSEM (deconv/ProcessForDeconvolution.java:372-405)
+  if (osem != 1.0 && (mode == MVD_WEIGHTS_PRECOMPUTED || mode == MVD_WEIGHTS_AS_GIVEN))
+    MVD_CHECK(launch_osem_scale(s->d_weights.as<float>(), s->n * s->nviews, (float)osem, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  s->osem = osem;
+  s->weight_mode = mode;
+  if (min_avg_out) {
+    min_avg_out[0] = mn;
+    min_avg_out[1] = avg;
+  }
+  return 0;
+}
+
+int mvd_prepare(mvd_session* s, int psf_type, double lambda, const int* block_xyz, int norm_threads) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
+  MVD_REQUIRE(psf_type >= 0 && psf_type <= 3, "invalid PSF type");
+  MVD_REQUIRE(norm_threads >= 1, "norm_threads must be >= 1");
+  DeviceGuard guard(s->device);
+  s->psf_type = psf_type;
+  s->lambda = lambda;
+  s->norm_threads = norm_threads;
+  s->use_blocks = block_xyz != nullptr;
+  if (block_xyz) {
+    MVD_REQUIRE(block_xyz[0] > 0 && block_xyz[1] > 0 && block_xyz[2] > 0, "invalid block size");
+    s->block = {block_xyz[0], block_xyz[1], block_xyz[2]};
+  }
+  if (s->weight_mode < 0) {
+    bool all_w = true;
+    for (auto& v : s->views) all_w = all_w && v.has_weight;
+    s->weight_mode = all_w ? MVD_WEIGHTS_AS_GIVEN : MVD_WEIGHTS_NONE;
+  }
+  for (auto& v : s->views) v.k1 = v.psf_raw;
+  // MVDeconInput.init: views in order
+  for (int v = 0; v < s->nviews; ++v) {
+    int rc = init_view(s, v);
+    if (rc) return rc;
+  }
+  for (int v = 0; v < s->nviews; ++v) {
+    int rc = build_geometry(s, v);
+    if (rc) return rc;
+    rc = view_spectra(s, v);
+    if (rc) return rc;
+  }
+  int rc = ensure_workspace(s);
+  if (rc) return rc;
+  s->prepared = true;
+  return 0;
+}
+
+int mvd_set_psf(mvd_session* s, int view, const float* psf) {
+  MVD_REQUIRE(s && psf, "NULL argument");
+  MVD_REQUIRE(s->prepared, "mvd_prepare first");
+  MVD_REQUIRE(view >= 0 && view < s->nviews, "invalid view");
+  DeviceGuard guard(s->device);
+  auto& v = s->views[view];
+  v.psf_raw.assign(psf, psf + v.k1.size());
+  v.k1 = v.psf_raw;
+  int rc = init_view(s, view);
+  if (rc) return rc;
+  return view_spectra(s, view);
+}
+
+int mvd_init_psi(mvd_session* s, const float* psi0, double* avg_out) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
+  DeviceGuard guard(s->device);
+  if (psi0) {
+    MVD_CHECK(hipMemcpyAsync(s->d_psi.p, psi0, sizeof(float) * s->n, hipMemcpyHostToDevice, s->stream));
+    MVD_CHECK(hipStreamSynchronize(s->stream));
+    if (avg_out) *avg_out = std::nan("");
+  } else {
+    double sum = 0;
+    unsigned long long cnt = 0;
+    int rc = first_iteration_sums(s, nullptr, &sum, &cnt);
+    if (rc) return rc;
+    double avg = sum / (double)cnt;
+    if (std::isnan(avg)) avg = 0.5;  // MVDeconvolution.java:138-142
+    MVD_CHECK(launch_fill(s->d_psi.as<float>(), (float)avg, s->n, s->stream));
+    MVD_CHECK(hipStreamSynchronize(s->stream));
+    if (avg_out) *avg_out = avg;
+  }
+  s->psi_ready = true;
+  return 0;
+}
+
+int mvd_iterate(mvd_session* s, int n, double* stats) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE(s->prepared, "mvd_prepare first");
+  MVD_REQUIRE(s->psi_ready, "mvd_init_psi first");
+  MVD_REQUIRE(n >= 0, "negative iteration count");
+  if (n == 0) return 0;
+  DeviceGuard guard(s->device);
+  MVD_CHECK(s->d_stats.ensure(sizeof(double) * 2 * s->nviews * (size_t)n));
+  float* psi = s->d_psi.as<float>();
+  float* ratio = s->d_ratio.as<float>();
+  float2* spec = s->d_spec.as<float2>();
+  const int wm = s->weight_mode == MVD_WEIGHTS_NONE ? kWeightNone
+                 : s->weight_mode == MVD_WEIGHTS_VIRTUAL ? kWeightVirtual
+                                                         : kWeightArray;
+  for (int it = 0; it < n; ++it) {
+    for (int v = 0; v < s->nviews; ++v) {
+      auto& view = s->views[v];
+      const ShapePlan& sp = *view.plan;
+      // convolve1 + quotient: psi -> ratio (MVDeconvolution.java:383-406)
+      EpilogueArgs q{};
+      q.mode = kEpiQuotient;
+      q.dst = ratio;
+      q.img = s->d_views.as<float>() + v * s->n;
+      for (auto& g : view.blocks) {
+        int rc = run_conv(s, sp, g, psi, kGatherMirror, view.k1hat.as<float2>(), q, spec);
+        if (rc) return rc;
+      }
+      // convolve2 + update: ratio -> psi (MVDeconvolution.java:418-457)
+      EpilogueArgs u{};
+      u.mode = kEpiUpdate;
+      u.dst = psi;
+      u.weight_mode = wm;
+      u.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
+      u.wsum = s->d_wsum.as<float>();
+      u.osem = s->osem;
+      u.lambda = s->lambda;
+      int off = 0;
+      for (auto& g : view.blocks) {
+        u.part_sum = s->d_part_sum.as<double>() + off;
+        u.part_max = s->d_part_max.as<float>() + off;
+        int rc = run_conv(s, sp, g, ratio, kGatherOne, view.k2hat.as<float2>(), u, spec);
+        if (rc) return rc;
+        off += x_inverse_workgroups(sp, g);
+      }
+      s->timer.begin(s->stream);
+      MVD_CHECK(launch_reduce_stats(s->d_part_sum.as<double>(), s->d_part_max.as<float>(), off,
+                                    s->d_stats.as<double>() + 2 * ((size_t)it * s->nviews + v), s->stream));
+      s->timer.end(s->stream, kTReduce);
+    }
+  }
+  if (stats)
+    MVD_CHECK(hipMemcpyAsync(stats, s->d_stats.p, sizeof(double) * 2 * s->nviews * (size_t)n,
+                             hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  s->timer.collect();
+  return 0;
+}
+
+int mvd_get_psi(mvd_session* s, float* out, int masked) {
+  MVD_REQUIRE(s && out, "NULL argument");
+  MVD_REQUIRE(s->psi_ready, "mvd_init_psi first");
+  DeviceGuard guard(s->device);
+  MVD_CHECK(hipMemcpyAsync(out, s->d_psi.p, sizeof(float) * s->n, hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  if (masked) {
+    // counts of views with data into the ratio buffer (scratch between iterations)
+    double sum;
+    unsigned long long cnt;
+    int rc = first_iteration_sums(s, s->d_ratio.as<float>(), &sum, &cnt);
+    if (rc) return rc;
+    std::vector<float> counts(s->n);
+    MVD_CHECK(hipMemcpy(counts.data(), s->d_ratio.p, sizeof(float) * s->n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < s->n; ++i)
+      if (counts[i] == 0.f) out[i] = 0.f;
+  }
+  return 0;
+}
+
+int mvd_get_kernels(mvd_session* s, int view, float* kernel1, float* kernel2) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE(s->prepared, "mvd_prepare first");
+  MVD_REQUIRE(view >= 0 && view < s->nviews, "invalid view");
+  auto& v = s->views[view];
+  if (kernel1) std::copy(v.k1.begin(), v.k1.end(), kernel1);
+  if (kernel2) std::copy(v.k2.begin(), v.k2.end(), kernel2);
+  return 0;
+}
+
+int mvd_get_geometry(mvd_session* s, int view, int* out) {
+  MVD_REQUIRE(s && out, "NULL argument");
+  MVD_REQUIRE(s->prepared, "mvd_prepare first");
+  MVD_REQUIRE(view >= 0 && view < s->nviews, "invalid view");
+  auto& v = s->views[view];
+  out[0] = (int)v.blocks.size();
+  out[1] = v.plan->q.x;
+  out[2] = v.plan->q.y;
+  out[3] = v.plan->q.z;
+  return 0;
+}
+
+int mvd_set_timing(mvd_session* s, int on) {
+  MVD_REQUIRE(s, "NULL session");
+  s->timer.enable(on != 0);
+  return 0;
+}
+
+int mvd_get_timing(mvd_session* s, double* total_ms, long long* launches, int cap) {
+  MVD_REQUIRE(s, "NULL session");
+  for (int i = 0; i < kTNumClasses && i < cap; ++i) {
+    if (total_ms) total_ms[i] = s->timer.ms(i);
+    if (launches) launches[i] = s->timer.count(i);
+  }
+  return kTNumClasses;
+}
+
+const char* mvd_timing_name(int cls) {
+  if (cls < 0 || cls >= kTNumClasses) return "";
+  return kTimingNames[cls];
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""Python mirror of the reference's native-library glue (src/main/java/spim/process/cuda/).
+
+  CUDAFourierConvolution       -- the JNA interface (CUDAFourierConvolution.java:25-32,
+                                  CUDAStandardFunctions.java:26-45), calling the same
+                                  C symbols JNA binds, with Java-style arrays
+                                  (flat float32 / int32 numpy arrays, dims reversed).
+  Block, BlockGeneratorFixedSizePrecise
+                               -- block split, copy and paste (Block.java:47-423,
+                                  BlockGeneratorFixedSizePrecise.java:30-124) on host arrays.
+  convolve_blocks_jna          -- MVDeconFFTThreads.convolve{1,2}BlockCUDA
+                                  (MVDeconFFTThreads.java:73-115): the reference's own
+                                  per-block GPU loop, driven through the JNA symbols.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import numpy as np
+
+from . import _native
+
+OOB_MIRROR_SINGLE, OOB_VALUE_ONE, OOB_ZERO = 0, 1, 2
+
+
+class CUDAFourierConvolution:
+    """What `Native.load(path, CUDAFourierConvolution.class)` returns in the reference."""
+
+    def __init__(self):
+        self._L = _native.lib()
+
+    def getNumDevicesCUDA(self) -> int:
+        return self._L.getNumDevicesCUDA()
+
+    def getNameDeviceCUDA(self, devCUDA: int, name: bytearray) -> None:
+        buf = ctypes.create_string_buffer(256)
+        self._L.getNameDeviceCUDA(devCUDA, buf)
+        name[:256] = buf.raw[:256]
+
+    def getMemDeviceCUDA(self, devCUDA: int) -> int:
+        return self._L.getMemDeviceCUDA(devCUDA)
+
+    def getFreeMemDeviceCUDA(self, devCUDA: int) -> int:
+        return self._L.getFreeMemDeviceCUDA(devCUDA)
+
+    def getCUDAcomputeCapabilityMajorVersion(self, devCUDA: int) -> int:
+        return self._L.getCUDAcomputeCapabilityMajorVersion(devCUDA)
+
+    def getCUDAcomputeCapabilityMinorVersion(self, devCUDA: int) -> int:
+        return self._L.getCUDAcomputeCapabilityMinorVersion(devCUDA)
+
+    @staticmethod
+    def _arrays(im, imDim, kernel, kernelDim):
+        if im.dtype != np.float32 or not im.flags["C_CONTIGUOUS"]:
+            raise TypeError("im must be a contiguous float32 array (Java float[])")
+        kernel = np.ascontiguousarray(kernel, dtype=np.float32)
+        imDim = np.ascontiguousarray(imDim, dtype=np.int32)
+        kernelDim = np.ascontiguousarray(kernelDim, dtype=np.int32)
+        if im.size != int(np.prod(imDim, dtype=np.int64)) or kernel.size != int(np.prod(kernelDim, dtype=np.int64)):
+            raise ValueError("array length does not match its dimensions")
+        return im, imDim, kernel, kernelDim
+
+    def convolution3DfftCUDAInPlace(self, im, imDim, kernel, kernelDim, devCUDA: int) -> None:
+        im, imDim, kernel, kernelDim = self._arrays(im, imDim, kernel, kernelDim)
+        p = ctypes.c_void_p
+        self._L.convolution3DfftCUDAInPlace(im.ctypes.data_as(p), imDim.ctypes.data_as(p),
+                                            kernel.ctypes.data_as(p), kernelDim.ctypes.data_as(p), devCUDA)
+
+    def convolution3DfftCUDA(self, im, imDim, kernel, kernelDim, devCUDA: int):
+        im, imDim, kernel, kernelDim = self._arrays(im, imDim, kernel, kernelDim)
+        p = ctypes.c_void_p
+        r = self._L.convolution3DfftCUDA(im.ctypes.data_as(p), imDim.ctypes.data_as(p),
+                                         kernel.ctypes.data_as(p), kernelDim.ctypes.data_as(p), devCUDA)
+        if not r:
+            return None
+        try:
+            return np.ctypeslib.as_array(r, shape=(im.size,)).copy()
+        finally:
+            self._L.mvd_free(ctypes.cast(r, ctypes.c_void_p))
+
+
+def _mirror_single(p: np.ndarray, n: int) -> np.ndarray:
+    """imglib2 mirror-single out-of-bounds index map (period 2n-2)."""
+    if n == 1:
+        return np.zeros_like(p)
+    period = 2 * n - 2
+    q = np.mod(p, period)
+    return np.where(q < n, q, period - q)
+
+
+class Block:
+    """cuda/Block.java:47-423 (3D, x-fastest arrays indexed [z, y, x])."""
+
+    def __init__(self, block_size, offset, effective_size, effective_offset, effective_local_offset):
+        self.block_size = tuple(block_size)
+        self.offset = tuple(offset)
+        self.effective_size = tuple(effective_size)
+        self.effective_offset = tuple(effective_offset)
+        self.effective_local_offset = tuple(effective_local_offset)
+
+    def copy_block(self, source: np.ndarray, oob: int) -> np.ndarray:
+        """Block.copyBlock (:134-173) from source extended by `oob`."""
+        idx = []
+        inside = []
+        for d, ax in ((0, 2), (1, 1), (2, 0)):
+            n = source.shape[ax]
+            p = np.arange(self.offset[d], self.offset[d] + self.block_size[d])
+            inside.append((p >= 0) & (p < n))
+            idx.append(_mirror_single(p, n) if oob == OOB_MIRROR_SINGLE else np.clip(p, 0, n - 1))
+        blk = source[np.ix_(idx[2], idx[1], idx[0])].astype(np.float32)
+        if oob != OOB_MIRROR_SINGLE:
+            mask = inside[2][:, None, None] & inside[1][None, :, None] & inside[0][None, None, :]
+            blk = np.where(mask, blk, np.float32(1.0 if oob == OOB_VALUE_ONE else 0.0)).astype(np.float32)
+        return np.ascontiguousarray(blk)
+
+    def paste_block(self, target: np.ndarray, block: np.ndarray) -> None:
+        """Block.pasteBlock (:175-215): effective region only."""
+        lo, eo, es = self.effective_local_offset, self.effective_offset, self.effective_size
+        target[eo[2]:eo[2] + es[2], eo[1]:eo[1] + es[1], eo[0]:eo[0] + es[0]] = \
+            block[lo[2]:lo[2] + es[2], lo[1]:lo[1] + es[1], lo[0]:lo[0] + es[0]]
+
+
+class BlockGeneratorFixedSizePrecise:
+    """cuda/BlockGeneratorFixedSizePrecise.java:30-124."""
+
+    def __init__(self, block_size: Sequence[int]):
+        self.block_size = tuple(int(b) for b in block_size)
+
+    def divide_into_blocks(self, img_size: Sequence[int], kernel_size: Sequence[int]) -> List[Block]:
+        eff = [b - k + 1 for b, k in zip(self.block_size, kernel_size)]
+        if any(e <= 0 for e in eff):
+            raise ValueError("block size smaller than the kernel")
+        local = [k // 2 for k in kernel_size]
+        nb = [i // e + (1 if i % e else 0) for i, e in zip(img_size, eff)]
+        blocks = []
+        for k in range(nb[2]):
+            for j in range(nb[1]):
+                for i in range(nb[0]):
+                    cur = (i, j, k)
+                    eoff = [c * e for c, e in zip(cur, eff)]
+                    off = [eo - ks // 2 for eo, ks in zip(eoff, kernel_size)]
+                    esz = [e if eo + e <= n else n - eo for e, eo, n in zip(eff, eoff, img_size)]
+                    blocks.append(Block(self.block_size, off, esz, eoff, local))
+        return blocks
+
+
+def convolve_blocks_jna(cuda: CUDAFourierConvolution, image: np.ndarray, kernel: np.ndarray, oob: int,
+                        blocks: Sequence[Block], device: int = 0) -> np.ndarray:
+    """convolve1BlockCUDA / convolve2BlockCUDA (MVDeconFFTThreads.java:73-115) for all blocks."""
+    result = np.empty_like(image, dtype=np.float32)
+    kz, ky, kx = kernel.shape
+    kdim = np.array([kz, ky, kx], np.int32)  # getCUDACoordinates: reversed (:157-165)
+    kflat = np.ascontiguousarray(kernel, dtype=np.float32).ravel()
+    for b in blocks:
+        blk = b.copy_block(image, oob)
+        flat = blk.ravel()
+        bx, by, bz = b.block_size
+        cuda.convolution3DfftCUDAInPlace(flat, np.array([bz, by, bx], np.int32), kflat, kdim, device)
+        b.paste_block(result, flat.reshape(bz, by, bx))
+    return result

@@ -1,0 +1,264 @@
+"""Python mirror of the reference's deconvolution operator API, backed by the
+GPU session of libmvdecon_hip (include/mvdecon.h).
+
+Reference classes (src/main/java/spim/process/fusion/deconvolution/):
+  MVDeconFFT      -- one view: image, weight, kernel, device list, blocks
+                     (MVDeconFFT.java:79-172; getters :371-377; setKernel :364-369)
+  MVDeconInput    -- the list of views (MVDeconInput.java:31-80)
+  MVDeconvolution -- initial estimate, iterations, final mask
+                     (MVDeconvolution.java:94-211; runIteration :349-465)
+
+Volumes are numpy float32 arrays indexed [z, y, x] (x fastest, as the
+reference's ArrayImg); block sizes are given (x, y, z) as in the reference.
+Errors raise; there is no CPU fallback (device -1, the reference's CPU path,
+is rejected).
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _native
+
+
+class PSFTYPE(enum.IntEnum):
+    """MVDeconFFT.PSFTYPE (MVDeconFFT.java:49), same order."""
+    OPTIMIZATION_II = 0
+    OPTIMIZATION_I = 1
+    EFFICIENT_BAYESIAN = 2
+    INDEPENDENT = 3
+
+
+class WeightType(enum.IntEnum):
+    """How per-view weights are used (ProcessForDeconvolution.java:221-243)."""
+    NO_WEIGHTS = 0
+    AS_GIVEN = 1             # already normalised by the caller
+    PRECOMPUTED_WEIGHTS = 2  # WeightNormalizer.ApplyDirectly on the GPU
+    VIRTUAL_WEIGHTS = 3      # NormalizingRandomAccess, evaluated per voxel
+
+
+def _f32(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 3:
+        raise ValueError("expected a 3D [z, y, x] array")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class MVDeconFFT:
+    """One view of the deconvolution (MVDeconFFT.java:79-172)."""
+
+    def __init__(self, image, weight, kernel, device_list: Sequence[int] = (0,), use_blocks: bool = False,
+                 block_size: Optional[Sequence[int]] = None, save_memory: bool = False):
+        self.image = _f32(image)
+        self.weight = None if weight is None else _f32(weight)
+        if self.weight is not None and self.weight.shape != self.image.shape:
+            raise ValueError("weight and image differ in size")
+        self.kernel1 = _f32(kernel)
+        if any(s % 2 == 0 for s in self.kernel1.shape):
+            raise ValueError("kernel sizes must be odd (BlockGeneratorFixedSizePrecise.java:43)")
+        self.device_list = list(device_list)
+        if not self.device_list or any(d < 0 for d in self.device_list):
+            raise ValueError("only GPU devices (>= 0) are supported; the reference's CPU path (-1) is not")
+        self.use_blocks = bool(use_blocks)
+        if self.use_blocks:
+            if block_size is None or len(block_size) != 3:
+                raise ValueError("use_blocks needs a 3D block size (x, y, z)")
+            self.block_size = tuple(int(b) for b in block_size)
+        else:
+            self.block_size = None
+        self.save_memory = save_memory  # kernel spectra always stay resident in HBM
+        self.kernel2: Optional[np.ndarray] = None
+        self.num_views = 0
+        self._session = None
+        self._index = -1
+
+    # getters of the reference (MVDeconFFT.java:371-377)
+    def get_image(self):
+        return self.image
+
+    def get_weight(self):
+        return self.weight
+
+    def get_kernel1(self):
+        return self.kernel1
+
+    def get_kernel2(self):
+        return self.kernel2
+
+    def set_num_views(self, n: int):
+        self.num_views = n
+
+    def set_kernel(self, kernel):
+        """MVDeconFFT.setKernel (:364-369): new PSF, re-run init for this view."""
+        k = _f32(kernel)
+        if k.shape != self.kernel1.shape:
+            raise ValueError("kernel size cannot change")
+        if self._session is None:
+            self.kernel1 = k
+            return
+        s = self._session
+        _native.check(_native.lib().mvd_set_psf(s.handle, self._index, _ptr(k)), "mvd_set_psf")
+        s._fetch_kernels()
+
+
+class MVDeconInput:
+    """List of views (MVDeconInput.java:31-80)."""
+
+    min_value = np.float32(0.0001)
+
+    def __init__(self):
+        self.views: List[MVDeconFFT] = []
+
+    def add(self, view: MVDeconFFT):
+        self.views.append(view)
+        for v in self.views:
+            v.set_num_views(len(self.views))
+
+    def get_views(self):
+        return self.views
+
+    def get_num_views(self):
+        return len(self.views)
+
+
+class _Session:
+    """Owns one mvd_session handle."""
+
+    def __init__(self, device: int, shape, nviews: int):
+        z, y, x = shape
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().mvd_create(device, x, y, z, nviews, ctypes.byref(h)), "mvd_create")
+        self.handle = h
+        self.shape = shape
+        self.nviews = nviews
+        self.views: List[MVDeconFFT] = []
+
+    def add(self, view: MVDeconFFT):
+        kz, ky, kx = view.kernel1.shape
+        _native.check(_native.lib().mvd_add_view(self.handle, _ptr(view.image), _ptr(view.weight),
+                                                 _ptr(view.kernel1), kx, ky, kz), "mvd_add_view")
+        view._session = self
+        view._index = len(self.views)
+        self.views.append(view)
+
+    def _fetch_kernels(self):
+        for i, v in enumerate(self.views):
+            k1 = np.empty_like(v.kernel1)
+            k2 = np.empty_like(v.kernel1)
+            _native.check(_native.lib().mvd_get_kernels(self.handle, i, _ptr(k1), _ptr(k2)), "mvd_get_kernels")
+            v.kernel1, v.kernel2 = k1, k2
+
+    def close(self):
+        if self.handle:
+            _native.lib().mvd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MVDeconvolution:
+    """MVDeconvolution (MVDeconvolution.java:94-211): the constructor runs the
+    whole deconvolution, like the reference; `get_psi()` returns the masked
+    estimate.  `run=False` only prepares (kernels, initial estimate) so that
+    `run_iteration()` can be driven step by step."""
+
+    min_value = np.float32(0.0001)
+
+    def __init__(self, views: MVDeconInput, iteration_type: PSFTYPE, num_iterations: int, lam: float,
+                 osem_speedup: float = 1.0, osem_speedup_index: int = 0, name: str = "deconvolved",
+                 norm_threads: int = 8, weight_type: Optional[WeightType] = None,
+                 initial_psi=None, run: bool = True):
+        vs = views.get_views()
+        if not vs:
+            raise ValueError("no views")
+        shape = vs[0].image.shape
+        for v in vs:
+            if v.image.shape != shape:
+                raise ValueError("all views must have the same size")
+        v0 = vs[0]
+        self.name = name
+        self.views = views
+        self.num_iterations = int(num_iterations)
+        self.lam = float(lam)
+        self.i = 0
+        self.stats: List[np.ndarray] = []
+        self._s = _Session(v0.device_list[0], shape, len(vs))
+        for v in vs:
+            self._s.add(v)
+        if weight_type is None:
+            weight_type = WeightType.AS_GIVEN if all(v.weight is not None for v in vs) else WeightType.NO_WEIGHTS
+        self.weight_type = WeightType(weight_type)
+        mm = (ctypes.c_double * 2)()
+        L = _native.lib()
+        _native.check(L.mvd_set_weights(self._s.handle, int(self.weight_type), int(osem_speedup_index),
+                                        float(osem_speedup), int(norm_threads), mm), "mvd_set_weights")
+        self.min_overlapping_views, self.avg_overlapping_views = mm[0], mm[1]
+        blk = None
+        if v0.use_blocks:
+            blk = (ctypes.c_int * 3)(*v0.block_size)
+        _native.check(L.mvd_prepare(self._s.handle, int(iteration_type), self.lam,
+                                    ctypes.cast(blk, ctypes.c_void_p) if blk is not None else None,
+                                    int(norm_threads)), "mvd_prepare")
+        self._s._fetch_kernels()
+        avg = ctypes.c_double()
+        p0 = None if initial_psi is None else _f32(initial_psi)
+        _native.check(L.mvd_init_psi(self._s.handle, _ptr(p0), ctypes.byref(avg)), "mvd_init_psi")
+        self.average = avg.value
+        if run:
+            self.iterate(self.num_iterations)
+
+    # MVDeconvolution.runIteration (:349-352)
+    def run_iteration(self):
+        self.iterate(1)
+
+    def iterate(self, n: int):
+        if n <= 0:
+            return
+        nv = self._s.nviews
+        st = np.zeros((n, nv, 2), np.float64)
+        _native.check(_native.lib().mvd_iterate(self._s.handle, n, _ptr(st)), "mvd_iterate")
+        self.stats.extend(list(st))
+        self.i += n
+
+    def get_current_iteration(self):
+        return self.i
+
+    def get_data(self):
+        return self.views
+
+    def get_name(self):
+        return self.name
+
+    def get_psi(self, masked: bool = True) -> np.ndarray:
+        out = np.empty(self._s.shape, np.float32)
+        _native.check(_native.lib().mvd_get_psi(self._s.handle, _ptr(out), 1 if masked else 0), "mvd_get_psi")
+        return out
+
+    def geometry(self, view: int = 0):
+        g = (ctypes.c_int * 4)()
+        _native.check(_native.lib().mvd_get_geometry(self._s.handle, view, g), "mvd_get_geometry")
+        return dict(blocks=g[0], fft=(g[1], g[2], g[3]))
+
+    def set_timing(self, on: bool):
+        _native.check(_native.lib().mvd_set_timing(self._s.handle, 1 if on else 0), "mvd_set_timing")
+
+    def timing(self):
+        L = _native.lib()
+        ms = (ctypes.c_double * 32)()
+        cnt = (ctypes.c_longlong * 32)()
+        n = L.mvd_get_timing(self._s.handle, ms, cnt, 32)
+        return {L.mvd_timing_name(i).decode(): (ms[i], cnt[i]) for i in range(n)}
+
+    def close(self):
+        self._s.close()

@@ -1,0 +1,207 @@
+"""GPU parity of libmvdecon_hip against the oracle (float32 restatement of the
+reference path, plus its FP64 twin), through the C ABI.
+
+Gate (deconvolved volume): max|psi_gpu - psi_ref| / max|psi_ref| <= 1e-4.
+Convolutions and kernels: float32 rounding budget, stated per test.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spim_registration_amd import MVDeconFFT, MVDeconInput, MVDeconvolution, PSFTYPE, WeightType
+from spim_registration_amd import cuda as C
+from spim_registration_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+GATE = 1e-4
+
+
+def rel_max(a, b):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / np.abs(b.astype(np.float64)).max())
+
+
+@pytest.fixture(scope="module")
+def jna():
+    return C.CUDAFourierConvolution()
+
+
+def test_device_query(jna):
+    n = jna.getNumDevicesCUDA()
+    assert n >= 1
+    name = bytearray(256)
+    jna.getNameDeviceCUDA(0, name)
+    assert name.rstrip(b"\0")
+    assert jna.getMemDeviceCUDA(0) > 100 * 2**30
+    assert 0 < jna.getFreeMemDeviceCUDA(0) <= jna.getMemDeviceCUDA(0)
+    assert jna.getCUDAcomputeCapabilityMajorVersion(0) >= 9
+    assert jna.getCUDAcomputeCapabilityMajorVersion(n + 5) == -1
+
+
+@pytest.mark.parametrize("bshape,kshape", [((20, 24, 32), (3, 7, 5)), ((94, 146, 146), (31, 19, 19)),
+                                           ((16, 30, 50), (5, 5, 9)), ((64, 64, 64), (13, 11, 9))])
+def test_jna_inplace_interior_matches_linear_convolution(jna, bshape, kshape):
+    rng = np.random.default_rng(sum(bshape))
+    blk = rng.random(bshape).astype(np.float32)
+    k = rng.random(kshape).astype(np.float32)
+    k /= k.sum()
+    print("jna block", bshape, "kernel", kshape, flush=True)
+    ref = O.fft_convolve(blk.astype(np.float64), k.astype(np.float64), O.OOB_ZERO)
+    flat = blk.ravel().copy()
+    jna.convolution3DfftCUDAInPlace(flat, np.array(bshape, np.int32), k.ravel(), np.array(kshape, np.int32), 0)
+    out = flat.reshape(bshape)
+    c = [s // 2 for s in kshape]
+    inner = tuple(slice(ci, s - ci) for ci, s in zip(c, bshape))
+    assert rel_max(out[inner], ref[inner]) < 2e-6
+    # the non-in-place variant gives the same result and leaves the input alone
+    before = blk.ravel().copy()
+    res = jna.convolution3DfftCUDA(before, np.array(bshape, np.int32), k.ravel(), np.array(kshape, np.int32), 0)
+    np.testing.assert_array_equal(before, blk.ravel())
+    np.testing.assert_array_equal(res, flat)
+
+
+def test_jna_invalid_input_leaves_block_unchanged(jna):
+    blk = np.ones(1000, np.float32)
+    jna.convolution3DfftCUDAInPlace(blk, np.array([10, 10, 10], np.int32), np.ones(8, np.float32),
+                                    np.array([2, 2, 2], np.int32), 99)  # bad device
+    assert np.all(blk == 1)
+
+
+@pytest.mark.parametrize("oob", [C.OOB_MIRROR_SINGLE, C.OOB_VALUE_ONE])
+def test_jna_blocked_convolution_matches_oracle(jna, oob):
+    """The reference's own per-block GPU loop (MVDeconFFTThreads.convolve*BlockCUDA)
+    through the JNA symbols equals the oracle's blocked CPU convolution."""
+    rng = np.random.default_rng(5)
+    img = (100 + 50 * rng.random((40, 50, 60))).astype(np.float32)
+    k = S.psf((9, 7, 11), 1.2, 2.5, 30)
+    blocks = C.BlockGeneratorFixedSizePrecise((32, 30, 24)).divide_into_blocks((60, 50, 40), (9, 7, 11))
+    out = C.convolve_blocks_jna(jna, img, k, oob, blocks)
+    ref = O.convolve(img, k, O.OOB_MIRROR if oob == C.OOB_MIRROR_SINGLE else O.OOB_ONE, block_xyz=(32, 30, 24))
+    assert rel_max(out, ref) < 2e-6
+
+
+def _session(views, psfs, ptype, iters, lam, weights=None, block=None, wtype=None, osem=1.0, osem_index=0,
+             norm_threads=8, psi0=None):
+    inp = MVDeconInput()
+    for i, (v, p) in enumerate(zip(views, psfs)):
+        w = None if weights is None else weights[i]
+        inp.add(MVDeconFFT(v, w, p, [0], block is not None, block))
+    return MVDeconvolution(inp, ptype, iters, lam, osem, osem_index, norm_threads=norm_threads, weight_type=wtype,
+                           initial_psi=psi0)
+
+
+@pytest.mark.parametrize("ptype", list(PSFTYPE))
+def test_kernel_preparation_matches_oracle(ptype):
+    psfs = [S.psf((9, 11, 13), 1.2, 3.0, a) for a in (0, 120, 240)]
+    views = [np.full((20, 24, 28), 100, np.float32) for _ in psfs]
+    print("kernels", ptype.name, flush=True)
+    d = _session(views, psfs, ptype, 0, 0.0, norm_threads=8)
+    k1, k2 = O.init_kernels(psfs, int(ptype), 8)
+    for v in range(3):
+        vv = d.get_data().get_views()[v]
+        np.testing.assert_array_equal(vv.get_kernel1(), k1[v])  # host normalisation is exact
+        assert rel_max(vv.get_kernel2(), k2[v]) < 2e-5
+    d.close()
+
+
+def _small_dataset(seed=7, size=(40, 36, 24), nv=3):
+    return S.make_dataset(size, nv, [0, 120, 240, 60, 180, 300][:nv], 30, (7, 7, 9), 1.2, 2.5, seed=seed)
+
+
+@pytest.mark.parametrize("ptype,lam,block", [
+    (PSFTYPE.EFFICIENT_BAYESIAN, 0.0, None),
+    (PSFTYPE.EFFICIENT_BAYESIAN, 0.006, None),
+    (PSFTYPE.OPTIMIZATION_I, 0.006, (24, 20, 16)),
+    (PSFTYPE.INDEPENDENT, 0.0, (30, 30, 20)),
+    (PSFTYPE.OPTIMIZATION_II, 0.004, (40, 36, 24)),
+])
+def test_deconvolution_matches_oracle(ptype, lam, block):
+    ds = _small_dataset()
+    print("deconv", ptype.name, lam, block, flush=True)
+    d = _session(ds.views, ds.psfs, ptype, 6, lam, block=block)
+    psi = d.get_psi()
+    ref, st = O.deconvolve(ds.views, ds.psfs, int(ptype), 6, lam, block_xyz=block, norm_threads=8)
+    assert rel_max(psi, ref) < GATE
+    gst = np.array(d.stats)
+    np.testing.assert_allclose(gst[:, :, 0], st[:, :, 0], rtol=1e-3)
+    np.testing.assert_allclose(gst[:, :, 1], st[:, :, 1], rtol=1e-2)
+    assert abs(d.average - O.first_iteration(ds.views, 8)[0]) < 1e-9 * d.average
+    d.close()
+
+
+@pytest.mark.parametrize("wtype,osem_index", [(WeightType.AS_GIVEN, 0), (WeightType.PRECOMPUTED_WEIGHTS, 1),
+                                              (WeightType.VIRTUAL_WEIGHTS, 2), (WeightType.PRECOMPUTED_WEIGHTS, 3)])
+def test_weighted_deconvolution_matches_oracle(wtype, osem_index):
+    ds = _small_dataset(seed=9)
+    rng = np.random.default_rng(3)
+    raw = [(rng.random(ds.views[0].shape) * (rng.random(ds.views[0].shape) > 0.2)).astype(np.float32)
+           for _ in ds.views]
+    raw[0] = np.maximum(raw[0], np.float32(0.05))  # every voxel keeps some weight (no 0/0)
+    osem_manual = 1.7
+    print("weights", wtype.name, osem_index, flush=True)
+    d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 4, 0.006, weights=raw, wtype=wtype,
+                 osem=osem_manual, osem_index=osem_index)
+    psi = d.get_psi()
+    # host restatement of WeightNormalizer + adjustForOSEM for the oracle
+    w = np.stack(raw).astype(np.float64)
+    s = w.sum(axis=0)
+    cnt = (w > 0).sum(axis=0)
+    parts = O.divide_portions(cnt.size, 16)
+    flat = cnt.ravel()
+    mn = max(1, int(flat.min()))
+    avg = max(1.0, float(np.mean([flat[a:a + n].sum() / n for a, n in parts])))
+    osem = {0: 1.0, 1: float(mn), 2: avg, 3: osem_manual}[osem_index]
+    if wtype == WeightType.AS_GIVEN:
+        ow = [np.minimum(np.float32(1), x * np.float32(osem)) if osem != 1 else x for x in raw]
+    elif wtype == WeightType.PRECOMPUTED_WEIGHTS:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            nw = [(x.astype(np.float64) / s).astype(np.float32) for x in raw]
+        ow = [np.minimum(np.float32(1), x * np.float32(osem)) for x in nw] if osem != 1 else nw
+    else:
+        ss = np.where(s > 1, s.astype(np.float32), np.float32(1)).astype(np.float64)
+        ow = [np.minimum(1.0, x.astype(np.float64) / ss * osem).astype(np.float32) for x in raw]
+    assert abs(d.min_overlapping_views - (mn if wtype != WeightType.AS_GIVEN else 1)) < 1e-9
+    ref, _ = O.deconvolve(ds.views, ds.psfs, O.EFFICIENT_BAYESIAN, 4, 0.006, weights=ow, norm_threads=8)
+    assert rel_max(psi, ref) < GATE
+    d.close()
+
+
+def test_initial_psi_and_resume():
+    ds = _small_dataset(seed=4)
+    d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 2, 0.0)
+    mid = d.get_psi(masked=False)
+    d.iterate(2)
+    full = d.get_psi()
+    d.close()
+    # resuming from the estimate after 2 iterations gives the same as 4 in a row
+    d2 = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 2, 0.0, psi0=mid)
+    np.testing.assert_array_equal(d2.get_psi(), full)
+    d2.close()
+
+
+def test_set_kernel_reinitialises_view():
+    ds = _small_dataset(seed=5)
+    d = _session(ds.views, ds.psfs, PSFTYPE.INDEPENDENT, 1, 0.0)
+    newk = S.psf((7, 7, 9), 1.0, 2.0, 45)
+    d.get_data().get_views()[1].set_kernel(newk)
+    k1, k2 = O.init_kernels([newk], O.INDEPENDENT, 8)
+    np.testing.assert_array_equal(d.get_data().get_views()[1].get_kernel1(), k1[0])
+    np.testing.assert_array_equal(d.get_data().get_views()[1].get_kernel2(), k2[0])
+    d.close()
+
+
+def test_config1_parity():
+    """Config 1: 3 views of 128x128x64, PSF 19x19x31, 10 iterations, lambda 0,
+    whole image as one block; against the float32 oracle (gate) and the FP64
+    oracle (reported)."""
+    ds, w = S.workload("c1")
+    print("c1 gpu", flush=True)
+    d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, w["iterations"], w["lam"])
+    psi = d.get_psi()
+    assert d.geometry(0)["blocks"] == 1
+    d.close()
+    print("c1 oracle", flush=True)
+    ref, _ = O.deconvolve(ds.views, ds.psfs, O.EFFICIENT_BAYESIAN, w["iterations"], w["lam"], norm_threads=8)
+    err = rel_max(psi, ref)
+    print("c1 rel err vs f32 oracle", err, flush=True)
+    assert err < GATE
