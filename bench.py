@@ -1,0 +1,214 @@
+"""Benchmark of the MI355X multiview deconvolution (plain method, config 2).
+
+Workload (one GPU): 6 views of 512x512x256 float32, PSF 31x31x51
+(sxy 1.2, sz 4.0, views every 60 deg about y), 500 seeded beads on a
+background of 100, Poisson noise; blending weights normalised on the GPU
+(precomputed weights); efficient-Bayesian kernels; blocks of 256^3 with
+PSF-sized overlap (18 blocks per view); Tikhonov lambda 0.006.
+
+A step is one deconvolution iteration (every view, every block).  Inputs,
+weights, estimate and kernel spectra are resident in HBM before the timed
+region.  Prints one JSON line.
+
+Roofline: the dominant kernel's algorithmic bytes per launch over its
+average duration (HIP events on the session stream), and the whole path's
+algorithmic bytes (8P + 48H + 20E per block, view and iteration) over the
+measured step time.  CPU baseline: the oracle (float32 C++ restatement of
+the reference CPU path) on one view-iteration of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gvoxel·view-iters/s MV-deconv (batched vs plain) + % HBM roofline at 1e-4"
+UNIT = "Gvoxel·view-iter/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(geom_blocks, q, img_n):
+    """Algorithmic HBM bytes of each pass of one view (all blocks), per class."""
+    qx, qy, qz = q
+    hx = qx // 2 + 1
+    H = hx * qy * qz
+    P = qx * qy * qz
+    b = dict(x_forward=0.0, y_forward=0.0, z_conv=0.0, y_inverse=0.0, x_inverse_quotient=0.0,
+             x_inverse_update=0.0)
+    for (ex, ey, ez) in geom_blocks:
+        E = ex * ey * ez
+        b["x_forward"] += 4 * P + 8 * H
+        b["y_forward"] += 16 * H
+        b["z_conv"] += 8 * H + 8 * H + 8 * H * ez / qz
+        b["y_inverse"] += 8 * H * ez / qz + 8 * H * ez * ey / (qz * qy)
+        rows = hx * ey * ez * 8
+        b["x_inverse_quotient"] += rows + 8 * E
+        b["x_inverse_update"] += rows + 12 * E
+    return b
+
+
+def path_bytes(geom_blocks, q):
+    qx, qy, qz = q
+    P = qx * qy * qz
+    H = (qx // 2 + 1) * qy * qz
+    return sum(8 * P + 48 * H + 20 * ex * ey * ez for (ex, ey, ez) in geom_blocks)
+
+
+def block_effective_sizes(img, ker, bs):
+    effs = []
+    eff = [b - k + 1 for b, k in zip(bs, ker)]
+    nb = [i // e + (1 if i % e else 0) for i, e in zip(img, eff)]
+    for k in range(nb[2]):
+        for j in range(nb[1]):
+            for i in range(nb[0]):
+                sz = []
+                for d, c in enumerate((i, j, k)):
+                    eo = c * eff[d]
+                    sz.append(eff[d] if eo + eff[d] <= img[d] else img[d] - eo)
+                effs.append(tuple(sz))
+    return effs
+
+
+def cpu_baseline(ds, w, threads):
+    """Oracle float32 path: one view-iteration of the workload (all 18 blocks)."""
+    from oracle import oracle as O
+    views, psfs = ds.views[:1], ds.psfs[:1]
+    blk = w["block"]
+    t0 = time.perf_counter()
+    O.deconvolve(views, psfs, O.EFFICIENT_BAYESIAN, 0, w["lam"], block_xyz=blk, norm_threads=8, threads=threads)
+    t1 = time.perf_counter()
+    O.deconvolve(views, psfs, O.EFFICIENT_BAYESIAN, 1, w["lam"], block_xyz=blk, norm_threads=8, threads=threads)
+    t2 = time.perf_counter()
+    it = (t2 - t1) - (t1 - t0)
+    n = int(np.prod(w["size_xyz"]))
+    return dict(value=n / it / 1e9, unit=UNIT, cores=threads, kind="port",
+                sample=f"oracle float32 CPU path, 1 view x 1 iteration of the 512x512x256 workload "
+                       f"(18 blocks of 256^3), {it:.2f} s, setup excluded")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    from spim_registration_amd import MVDeconFFT, MVDeconInput, MVDeconvolution, PSFTYPE, WeightType
+    from spim_registration_amd import synthetic as S
+
+    t = time.perf_counter()
+    ds, w = S.workload(args.workload)
+    weights = [S.blending_weight(v.shape) for v in ds.views]
+    log(f"generated {args.workload} in {time.perf_counter() - t:.1f} s")
+
+    inp = MVDeconInput()
+    blk = w["block"]
+    for v, p, wt in zip(ds.views, ds.psfs, weights):
+        inp.add(MVDeconFFT(v, wt, p, [0], blk is not None, blk))
+    t = time.perf_counter()
+    d = MVDeconvolution(inp, PSFTYPE.EFFICIENT_BAYESIAN, 0, w["lam"], 1.0, 0, norm_threads=8,
+                        weight_type=WeightType.PRECOMPUTED_WEIGHTS, run=False)
+    log(f"prepared in {time.perf_counter() - t:.1f} s; geometry {d.geometry(0)}")
+
+    d.iterate(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.iterate(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    n = int(np.prod(w["size_xyz"]))
+    nv = w["nviews"]
+    value = n * nv * args.steps / sec / 1e9
+    log(f"{args.steps} iterations in {sec:.3f} s -> {value:.2f} {UNIT}")
+
+    # per-kernel timing with events (separate, untimed pass)
+    d.set_timing(True)
+    d.iterate(1)
+    tim = d.timing()
+    d.set_timing(False)
+    geo = d.geometry(0)
+    q = geo["fft"]
+    effs = block_effective_sizes(w["size_xyz"], w["k_xyz"], blk) if blk else [tuple(w["size_xyz"])]
+    kb = kernel_bytes(effs, q, n)
+    per_class = {}
+    for name, (ms, cnt) in tim.items():
+        if cnt and name in kb:
+            bytes_total = kb[name] * nv  # one iteration = nv views
+            per_class[name] = dict(ms_total=ms, launches=cnt, avg_us=1e3 * ms / cnt,
+                                   gbs=bytes_total / (ms * 1e-3) / 1e9)
+    dom = max(per_class, key=lambda k: per_class[k]["ms_total"])
+    dc = per_class[dom]
+    achieved = dc["gbs"]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(dom)
+        except Exception:
+            traffic = None
+    pb = path_bytes(effs, q) * nv * args.steps
+    path_gbs = pb / sec / 1e9
+    log("per-kernel: " + json.dumps(per_class, indent=None))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        t = time.perf_counter()
+        cpu = cpu_baseline(ds, w, threads)
+        log(f"cpu baseline {cpu['value']:.4f} {UNIT} ({time.perf_counter() - t:.1f} s)")
+    d.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * sec / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {nv} views of {w['size_xyz'][0]}x{w['size_xyz'][1]}x"
+                               f"{w['size_xyz'][2]} f32, PSF {w['k_xyz'][0]}x{w['k_xyz'][1]}x{w['k_xyz'][2]}, "
+                               f"blocks {blk}, lambda {w['lam']}, efficient-Bayesian, precomputed weights",
+                   "views": nv, "size_xyz": list(w["size_xyz"]), "block": list(blk) if blk else None,
+                   "fft_shape": list(q), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_launch_us": round(dc["avg_us"], 2)},
+        "path_roofline": {"achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(path_gbs / HBM_PEAK_GBS, 4),
+                          "model": "8P + 48H + 20E bytes per block, view and iteration"},
+        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "GB/s": round(v["gbs"], 1),
+                        "share": round(v["ms_total"] / sum(x["ms_total"] for x in per_class.values()), 3)}
+                    for k, v in per_class.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -178,22 +178,30 @@ inline T extended(const T* in, Dims d, Oob oob, int64_t x, int64_t y, int64_t z)
 }
 
 // FFTConvolution (imglib2-algorithm) as called by the reference:
-// out(x) = sum_j E(in)(x + k/2 - j) * ker(j), out has in's size.
+// out(x) = sum_j E(in)(x + k/2 - j) * ker(j), out has in's size.  The kernel
+// spectrum for a given input size is computed once and reused, as
+// FFTConvolution keeps its kernel FFT between calls (MVDeconFFT.java:392-399,
+// 413-419).
 template <class T>
-void fft_convolve(const T* in, Dims din, Oob oob, const T* ker, Dims dk, T* out, int threads) {
-  using C = std::complex<T>;
-  const int64_t qx = next_fast_size(din.x + dk.x - 1, true);
-  const int64_t qy = next_fast_size(din.y + dk.y - 1, false);
-  const int64_t qz = next_fast_size(din.z + dk.z - 1, false);
-  const int64_t lx = (qx - din.x) / 2, ly = (qy - din.y) / 2, lz = (qz - din.z) / 2;
-  const int64_t q = qx * qy * qz;
-  std::vector<T> pad(q), kpad(q, (T)0);
-  parallel_for(qz, threads, [&](int64_t zb, int64_t ze) {
-    for (int64_t z = zb; z < ze; ++z)
-      for (int64_t y = 0; y < qy; ++y)
-        for (int64_t x = 0; x < qx; ++x)
-          pad[x + qx * (y + qy * z)] = extended(in, din, oob, x - lx, y - ly, z - lz);
-  });
+struct KernelFFT {
+  Dims din{0, 0, 0};
+  int64_t qx = 0, qy = 0, qz = 0, lx = 0, ly = 0, lz = 0;
+  std::unique_ptr<RealFFT3D<T>> fft;
+  std::vector<std::complex<T>> spec;
+};
+
+template <class T>
+KernelFFT<T> make_kernel_fft(Dims din, const T* ker, Dims dk, int threads) {
+  KernelFFT<T> kf;
+  kf.din = din;
+  kf.qx = next_fast_size(din.x + dk.x - 1, true);
+  kf.qy = next_fast_size(din.y + dk.y - 1, false);
+  kf.qz = next_fast_size(din.z + dk.z - 1, false);
+  kf.lx = (kf.qx - din.x) / 2;
+  kf.ly = (kf.qy - din.y) / 2;
+  kf.lz = (kf.qz - din.z) / 2;
+  const int64_t qx = kf.qx, qy = kf.qy, qz = kf.qz;
+  std::vector<T> kpad(qx * qy * qz, (T)0);
   const int64_t cx = dk.x / 2, cy = dk.y / 2, cz = dk.z / 2;
   for (int64_t z = 0; z < dk.z; ++z)
     for (int64_t y = 0; y < dk.y; ++y)
@@ -202,17 +210,34 @@ void fft_convolve(const T* in, Dims din, Oob oob, const T* ker, Dims dk, T* out,
                       hz = ((z - cz) % qz + qz) % qz;
         kpad[hx + qx * (hy + qy * hz)] = ker[x + dk.x * (y + dk.y * z)];
       }
-  RealFFT3D<T> fft((int)qx, (int)qy, (int)qz, threads);
-  std::vector<C> si(fft.spectrum_size()), sk(fft.spectrum_size());
-  fft.forward(pad.data(), si.data());
-  fft.forward(kpad.data(), sk.data());
+  kf.fft = std::make_unique<RealFFT3D<T>>((int)qx, (int)qy, (int)qz, threads);
+  kf.spec.resize(kf.fft->spectrum_size());
+  kf.fft->forward(kpad.data(), kf.spec.data());
+  return kf;
+}
+
+template <class T>
+void fft_convolve(const T* in, Dims din, Oob oob, const KernelFFT<T>& kf, T* out, int threads) {
+  using C = std::complex<T>;
+  const int64_t qx = kf.qx, qy = kf.qy, qz = kf.qz, lx = kf.lx, ly = kf.ly, lz = kf.lz;
+  const int64_t q = qx * qy * qz;
+  std::vector<T> pad(q);
+  parallel_for(qz, threads, [&](int64_t zb, int64_t ze) {
+    for (int64_t z = zb; z < ze; ++z)
+      for (int64_t y = 0; y < qy; ++y)
+        for (int64_t x = 0; x < qx; ++x)
+          pad[x + qx * (y + qy * z)] = extended(in, din, oob, x - lx, y - ly, z - lz);
+  });
+  std::vector<C> si(kf.fft->spectrum_size());
+  kf.fft->forward(pad.data(), si.data());
+  const std::vector<C>& sk = kf.spec;
   parallel_for((int64_t)si.size(), threads, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; ++i) {
       const C a = si[i], h = sk[i];
       si[i] = C(a.real() * h.real() - a.imag() * h.imag(), a.real() * h.imag() + a.imag() * h.real());
     }
   });
-  fft.inverse(si.data(), pad.data());
+  kf.fft->inverse(si.data(), pad.data());
   const T scale = (T)(1.0L / (long double)q);
   parallel_for(din.z, threads, [&](int64_t zb, int64_t ze) {
     for (int64_t z = zb; z < ze; ++z)
@@ -220,6 +245,12 @@ void fft_convolve(const T* in, Dims din, Oob oob, const T* ker, Dims dk, T* out,
         for (int64_t x = 0; x < din.x; ++x)
           out[x + din.x * (y + din.y * z)] = pad[(x + lx) + qx * ((y + ly) + qy * (z + lz))] * scale;
   });
+}
+
+template <class T>
+void fft_convolve(const T* in, Dims din, Oob oob, const T* ker, Dims dk, T* out, int threads) {
+  KernelFFT<T> kf = make_kernel_fft(din, ker, dk, threads);
+  fft_convolve(in, din, oob, kf, out, threads);
 }
 
 // Brute-force FP64 direct convolution with the same definition.
@@ -278,26 +309,34 @@ static std::vector<BlockGeo> divide_into_blocks(Dims img, Dims ker, Dims bs) {
 // mirror-single extension (FFTConvolution(block, kernel)), and its effective
 // region pasted into out (Block.pasteBlock).
 template <class T>
-void convolve(const T* in, Dims d, Oob oob, const T* ker, Dims dk, const Dims* bs, T* out, int threads) {
+void convolve(const T* in, Dims d, Oob oob, const KernelFFT<T>& kf, Dims dk, const Dims* bs, T* out, int threads) {
   if (!bs) {
-    fft_convolve(in, d, oob, ker, dk, out, threads);
+    fft_convolve(in, d, oob, kf, out, threads);
     return;
   }
   auto blocks = divide_into_blocks(d, dk, *bs);
   std::vector<T> blk(bs->n()), res(bs->n());
   for (auto& g : blocks) {
-    for (int64_t z = 0; z < bs->z; ++z)
-      for (int64_t y = 0; y < bs->y; ++y)
-        for (int64_t x = 0; x < bs->x; ++x)
-          blk[x + bs->x * (y + bs->y * z)] =
-              extended(in, d, oob, x + g.offset[0], y + g.offset[1], z + g.offset[2]);
-    fft_convolve(blk.data(), *bs, OOB_MIRROR, ker, dk, res.data(), threads);
+    parallel_for(bs->z, threads, [&](int64_t zb, int64_t ze) {
+      for (int64_t z = zb; z < ze; ++z)
+        for (int64_t y = 0; y < bs->y; ++y)
+          for (int64_t x = 0; x < bs->x; ++x)
+            blk[x + bs->x * (y + bs->y * z)] =
+                extended(in, d, oob, x + g.offset[0], y + g.offset[1], z + g.offset[2]);
+    });
+    fft_convolve(blk.data(), *bs, OOB_MIRROR, kf, res.data(), threads);
     for (int64_t z = 0; z < g.eff_size[2]; ++z)
       for (int64_t y = 0; y < g.eff_size[1]; ++y)
         for (int64_t x = 0; x < g.eff_size[0]; ++x)
           out[(x + g.eff_offset[0]) + d.x * ((y + g.eff_offset[1]) + d.y * (z + g.eff_offset[2]))] =
               res[(x + g.eff_local[0]) + bs->x * ((y + g.eff_local[1]) + bs->y * (z + g.eff_local[2]))];
   }
+}
+
+template <class T>
+void convolve(const T* in, Dims d, Oob oob, const T* ker, Dims dk, const Dims* bs, T* out, int threads) {
+  KernelFFT<T> kf = make_kernel_fft(bs ? *bs : d, ker, dk, threads);
+  convolve(in, d, oob, kf, dk, bs, out, threads);
 }
 
 // MVDeconFFT.init, deconv/MVDeconFFT.java:183-324, for all views in order
@@ -450,14 +489,19 @@ void deconvolve(const RunArgs& a, const std::vector<const T*>& imgs, const std::
   }
   const Dims* bs = a.use_blocks ? &a.block : nullptr;
   auto portions = divide_into_portions(n, 2 * a.norm_threads);
+  std::vector<KernelFFT<T>> kf1, kf2;
+  for (int v = 0; v < a.nviews; ++v) {
+    kf1.push_back(make_kernel_fft(bs ? *bs : a.img, k1[v].data(), a.kdims[v], a.threads));
+    kf2.push_back(make_kernel_fft(bs ? *bs : a.img, k2[v].data(), a.kdims[v], a.threads));
+  }
   for (int it = 0; it < a.iterations; ++it) {
     for (int v = 0; v < a.nviews; ++v) {
-      convolve(psi, a.img, OOB_MIRROR, k1[v].data(), a.kdims[v], bs, tmp1.data(), a.threads);
+      convolve(psi, a.img, OOB_MIRROR, kf1[v], a.kdims[v], bs, tmp1.data(), a.threads);
       const T* img = imgs[v];
       parallel_for(n, a.threads, [&](int64_t b, int64_t e) {
         for (int64_t i = b; i < e; ++i) tmp1[i] = quotient(tmp1[i], img[i]);
       });
-      convolve(tmp1.data(), a.img, OOB_ONE, k2[v].data(), a.kdims[v], bs, tmp2.data(), a.threads);
+      convolve(tmp1.data(), a.img, OOB_ONE, kf2[v], a.kdims[v], bs, tmp2.data(), a.threads);
       const T* w = weights[v];
       std::vector<double> psum(portions.size()), pmax(portions.size());
       parallel_for((int64_t)portions.size(), a.threads, [&](int64_t pb, int64_t pe) {

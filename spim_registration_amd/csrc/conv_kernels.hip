@@ -330,6 +330,7 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
   const int M = g.q.x / 2;
   dim3 grid((nrows + P.lines - 1) / P.lines);
   size_t lds = sizeof(float2) * (g.q.x + (size_t)M * (P.lines + 1));
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_x_forward, grid, dim3(kThreads), lds, stream, P);
   return hipGetLastError();
 }
@@ -350,6 +351,7 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
   P.tw = sp.tw_y;
   dim3 grid((sp.hx + P.lines - 1) / P.lines, zhi - zlo);
   size_t lds = sizeof(float2) * (sp.q.y + (size_t)sp.q.y * (P.lines + 1));
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_y, grid, dim3(kThreads), lds, stream, P);
   return hipGetLastError();
 }
@@ -370,6 +372,7 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
   P.tw = sp.tw_z;
   dim3 grid((sp.hx + P.lines - 1) / P.lines, sp.q.y);
   size_t lds = sizeof(float2) * (sp.q.z + (size_t)sp.q.z * (P.lines + 1));
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_z, grid, dim3(kThreads), lds, stream, P);
   return hipGetLastError();
 }
@@ -394,6 +397,7 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
   if (nwg) *nwg = n;
   if (n == 0) return hipSuccess;
   size_t lds = sizeof(float2) * (g.q.x + (size_t)(M + 1) * (P.lines + 1));
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_x_inverse, dim3(n), dim3(kThreads), lds, stream, P);
   return hipGetLastError();
 }

@@ -141,7 +141,10 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
   if (!im || !kernel || !out) JNA_FAIL("NULL array");
   if (bd.x <= 0 || bd.y <= 0 || bd.z <= 0 || kd.x <= 0 || kd.y <= 0 || kd.z <= 0) JNA_FAIL("invalid dimensions");
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || dev < 0 || dev >= ndev) JNA_FAIL("invalid device id");
+  if (hipGetDeviceCount(&ndev) != hipSuccess || dev < 0 || dev >= ndev) {
+    (void)hipGetLastError();
+    JNA_FAIL("invalid device id");
+  }
   DeviceContext* c = context(dev);
   std::lock_guard<std::mutex> lk(c->mu);
   int prev = -1;
@@ -183,18 +186,26 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
     k.q = q;
     k.elems = sp->spectrum_elems();
     JNA_HIP(hipMalloc(&k.spec, sizeof(float2) * k.elems));
-    c->kcache.push_front(k);
-    ks = &c->kcache.front();
+    struct Guard {  // frees the spectrum unless it made it into the cache
+      float2** p;
+      ~Guard() {
+        if (*p) (void)hipFree(*p);
+      }
+    } guard{&k.spec};
     if (!grow((void**)&c->kern, &c->kern_cap, sizeof(float) * nk)) JNA_FAIL("out of device memory (kernel)");
     JNA_HIP(hipMemcpyAsync(c->kern, kernel, sizeof(float) * nk, hipMemcpyHostToDevice, c->stream));
-    ConvGeom g{};
-    g.src = kd;
-    g.win = q;
-    g.q = q;
-    JNA_HIP(launch_x_forward(*sp, g, c->kern, kGatherKernelWrap, ks->spec, c->stream));
-    JNA_HIP(launch_y(*sp, ks->spec, 0, q.z, 0, q.y, -1.f, c->stream));
+    ConvGeom kg{};
+    kg.src = kd;
+    kg.win = q;
+    kg.q = q;
+    JNA_HIP(launch_x_forward(*sp, kg, c->kern, kGatherKernelWrap, k.spec, c->stream));
+    JNA_HIP(launch_y(*sp, k.spec, 0, q.z, 0, q.y, -1.f, c->stream));
     const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
-    JNA_HIP(launch_z(*sp, ks->spec, nullptr, scale, 0, q.z, c->stream));
+    JNA_HIP(launch_z(*sp, k.spec, nullptr, scale, 0, q.z, c->stream));
+    JNA_HIP(hipStreamSynchronize(c->stream));
+    c->kcache.push_front(k);
+    k.spec = nullptr;
+    ks = &c->kcache.front();
   }
   JNA_HIP(hipMemcpyAsync(c->block, im, sizeof(float) * nb, hipMemcpyHostToDevice, c->stream));
   ConvGeom g{};
@@ -228,9 +239,15 @@ using namespace mvd;
 
 extern "C" {
 
+// Failed queries must not leave an error behind for later calls on the thread.
+static void clear_error() { (void)hipGetLastError(); }
+
 int getNumDevicesCUDA(void) {
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    clear_error();
+    return -1;
+  }
   return n;
 }
 
@@ -238,36 +255,56 @@ void getNameDeviceCUDA(int devCUDA, char* name) {
   if (!name) return;
   name[0] = 0;
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) {
+    clear_error();
+    return;
+  }
   std::strncpy(name, p.name, 255);
   name[255] = 0;
+  if (name[0] == 0) {
+    // some drivers report no marketing name; fall back to the ISA name
+    std::snprintf(name, 256, "AMD GPU %s", p.gcnArchName);
+  }
 }
 
 long long getMemDeviceCUDA(int devCUDA) {
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) {
+    clear_error();
+    return -1;
+  }
   return (long long)p.totalGlobalMem;
 }
 
 long long getFreeMemDeviceCUDA(int devCUDA) {
   int prev = -1;
   (void)hipGetDevice(&prev);
-  if (hipSetDevice(devCUDA) != hipSuccess) return -1;
+  if (hipSetDevice(devCUDA) != hipSuccess) {
+    clear_error();
+    return -1;
+  }
   size_t fr = 0, tot = 0;
   hipError_t e = hipMemGetInfo(&fr, &tot);
   if (prev >= 0) (void)hipSetDevice(prev);
+  if (e != hipSuccess) clear_error();
   return e == hipSuccess ? (long long)fr : -1;
 }
 
 int getCUDAcomputeCapabilityMajorVersion(int devCUDA) {
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) {
+    clear_error();
+    return -1;
+  }
   return p.major;
 }
 
 int getCUDAcomputeCapabilityMinorVersion(int devCUDA) {
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) {
+    clear_error();
+    return -1;
+  }
   return p.minor;
 }
 

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(kPwThreads) void k_first_iteration(const float* vie
 
 hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
                                   unsigned long long* part_cnt, int nparts, hipStream_t stream) {
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_first_iteration, dim3(nparts), dim3(kPwThreads), 0, stream, views, nviews, n, counts,
                      part_sum, part_cnt);
   return hipGetLastError();
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(kPwThreads) void k_fill(float* dst, float v, int64_
 hipError_t launch_fill(float* dst, float v, int64_t n, hipStream_t stream) {
   const int64_t blocks = std::min<int64_t>((n + kPwThreads - 1) / kPwThreads, 8192);
   if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(kPwThreads), 0, stream, dst, v, n);
   return hipGetLastError();
 }
@@ -82,6 +84,7 @@ __global__ __launch_bounds__(kPwThreads) void k_mask(const float* psi, const flo
 hipError_t launch_mask(const float* psi, const float* counts, float* out, int64_t n, hipStream_t stream) {
   const int64_t blocks = std::min<int64_t>((n + kPwThreads - 1) / kPwThreads, 8192);
   if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_mask, dim3((unsigned)blocks), dim3(kPwThreads), 0, stream, psi, counts, out, n);
   return hipGetLastError();
 }
@@ -116,6 +119,7 @@ __global__ __launch_bounds__(kPwThreads) void k_reduce_stats(const double* part_
 
 hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, int n, double* out,
                                hipStream_t stream) {
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(kPwThreads), 0, stream, part_sum, part_max, n, out);
   return hipGetLastError();
 }
@@ -169,6 +173,7 @@ __global__ __launch_bounds__(kPwThreads) void k_weight_normalize(float* weights,
 hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_t stride, float* wsum,
                                    int virtual_mode, int* part_min, unsigned long long* part_cnt, int nparts,
                                    hipStream_t stream) {
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_weight_normalize, dim3(nparts), dim3(kPwThreads), 0, stream, weights, nviews, n, stride, wsum,
                      virtual_mode, part_min, part_cnt);
   return hipGetLastError();
@@ -185,6 +190,7 @@ __global__ __launch_bounds__(kPwThreads) void k_osem_scale(float* w, int64_t n, 
 hipError_t launch_osem_scale(float* weights, int64_t n, float osem, hipStream_t stream) {
   const int64_t blocks = std::min<int64_t>((n + kPwThreads - 1) / kPwThreads, 8192);
   if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_osem_scale, dim3((unsigned)blocks), dim3(kPwThreads), 0, stream, weights, n, osem);
   return hipGetLastError();
 }
