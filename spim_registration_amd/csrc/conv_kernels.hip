@@ -317,6 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
 
 hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                             hipStream_t stream) {
+  if (fast_size(g.q.x / 2)) return launch_x_forward_fast(sp, g, src, gather, spec, stream);
   XFwdParams P;
   P.g = g;
   P.src = src;
@@ -338,6 +339,7 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
 hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                     hipStream_t stream) {
   if (zhi <= zlo) return hipSuccess;
+  if (fast_size(sp.q.y)) return launch_y_fast(sp, spec, zlo, zhi, ylo, yhi, dir, stream);
   YParams P;
   P.spec = spec;
   P.hx = sp.hx;
@@ -358,6 +360,7 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
 
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
                     hipStream_t stream) {
+  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream);
   ZParams P;
   P.spec = spec;
   P.kspec = kspec;
@@ -379,7 +382,8 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
 
 int x_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g) {
   const int nrows = g.eff_size.y * g.eff_size.z;
-  return (nrows + sp.bx - 1) / sp.bx;
+  const int rows = fast_size(g.q.x / 2) ? x_fast_rows_per_wg(g.q.x / 2) : sp.bx;
+  return (nrows + rows - 1) / rows;
 }
 
 hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
@@ -396,6 +400,7 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
   const int n = x_inverse_workgroups(sp, g);
   if (nwg) *nwg = n;
   if (n == 0) return hipSuccess;
+  if (fast_size(M)) return launch_x_inverse_fast(sp, g, spec, e, stream, n);
   size_t lds = sizeof(float2) * (g.q.x + (size_t)(M + 1) * (P.lines + 1));
   (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_x_inverse, dim3(n), dim3(kThreads), lds, stream, P);

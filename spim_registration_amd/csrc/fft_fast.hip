@@ -1,0 +1,622 @@
+// Register-resident FFT passes for power-of-two lengths (64, 128, 256, 512).
+//
+// Every line is transformed by T = N / E cooperating threads that each keep
+// E complex values in VGPRs.  A length is split into two or three radices
+// (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8); butterflies run
+// in registers and the lines meet in LDS only between radix passes -- one LDS
+// exchange for a 256-point line instead of one per radix-2/4 stage.  All
+// sizes are compile-time, so index arithmetic is shifts and masks.
+//
+// Stockham decimation in time: in a pass of radix R with Ns = product of the
+// earlier radices, butterfly j reads x[j + t N/R], multiplies by
+// exp(dir 2 pi i t (j mod Ns) / (Ns R)) and writes y[(j div Ns) Ns R + (j mod Ns)
+// + t Ns].  A thread owns butterflies j = jt + T b.  After the last pass the
+// thread holds X[jt + T b + t N/R_last], which is exactly the input order of a
+// transform with the radices reversed -- the fused z pass goes forward,
+// multiplies by the kernel spectrum and back without touching LDS in between.
+#include "mvd_internal.h"
+
+namespace mvd {
+
+template <>
+__device__ __forceinline__ void butterfly<16>(float2* v, float dir) {
+  // n = 4 n2 + n1, k = k1 + 4 k2
+  const float c1 = 0.923879532511286756128183f;  // cos(pi/8)
+  const float s1 = 0.382683432365089771728460f;  // sin(pi/8)
+  const float h = 0.707106781186547524400844f;
+  float2 a[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    float2 t[4] = {v[n1], v[4 + n1], v[8 + n1], v[12 + n1]};
+    butterfly<4>(t, dir);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) a[n1][k1] = t[k1];
+  }
+  // W16^m = cos(2 pi m / 16) + i dir sin(2 pi m / 16) for m = n1 k1
+  const float cs[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
+  const float sn[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
+#pragma unroll
+  for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) {
+      const int m = n1 * k1;
+      a[n1][k1] = cmul(a[n1][k1], make_float2(cs[m], dir * sn[m]));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    butterfly<4>(t, dir);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int N>
+struct Spec;
+template <>
+struct Spec<64> {
+  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1;
+};
+template <>
+struct Spec<128> {
+  static constexpr int E = 16, R0 = 16, R1 = 8, R2 = 1;
+};
+template <>
+struct Spec<256> {
+  static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1;
+};
+template <>
+struct Spec<512> {
+  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8;
+};
+
+// Butterflies of one pass on the registers (inputs already in pass order).
+template <int N, int E, int R, int NS>
+__device__ __forceinline__ void bfly_pass(float2 (&v)[E], int jt, const float2* tw, int tw_step, float dir) {
+  constexpr int T = N / E;
+  constexpr int TS = N / (NS * R);
+#pragma unroll
+  for (int b = 0; b < E / R; ++b) {
+    if constexpr (NS > 1) {
+      const int k = (jt + T * b) & (NS - 1);
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        float2 w = tw[t * k * TS * tw_step];
+        w.y *= -dir;
+        v[b * R + t] = cmul(v[b * R + t], w);
+      }
+    }
+    butterfly<R>(v + b * R, dir);
+  }
+}
+
+template <int N, int E, int R, int NS, class Addr>
+__device__ __forceinline__ void store_pass(const float2 (&v)[E], float2* lds, int jt, Addr addr) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int b = 0; b < E / R; ++b) {
+    const int j = jt + T * b;
+    const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+    for (int t = 0; t < R; ++t) lds[addr(base + t * NS)] = v[b * R + t];
+  }
+}
+
+template <int N, int E, int R, class Addr>
+__device__ __forceinline__ void load_pass(float2 (&v)[E], const float2* lds, int jt, Addr addr) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int b = 0; b < E / R; ++b)
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[b * R + t] = lds[addr(jt + T * b + t * (N / R))];
+}
+
+// Element index held in v[b * R + t] when the pass of radix R reads or the
+// last pass of radix R writes.
+template <int N, int E, int R>
+__device__ __forceinline__ int elem_index(int jt, int b, int t) {
+  return jt + (N / E) * b + t * (N / R);
+}
+
+// Full transform.  v: inputs in the order of the first radix A; on return the
+// outputs in the order of the last radix.  Starts and ends with a barrier, so
+// LDS may be in use by the caller before and after.
+template <int N, int E, int A, int B, int C, class Addr>
+__device__ __forceinline__ void fft_regs(float2 (&v)[E], float2* lds, int jt, const float2* tw, int tw_step,
+                                         float dir, Addr addr) {
+  bfly_pass<N, E, A, 1>(v, jt, tw, tw_step, dir);
+  __syncthreads();
+  store_pass<N, E, A, 1>(v, lds, jt, addr);
+  __syncthreads();
+  load_pass<N, E, B>(v, lds, jt, addr);
+  bfly_pass<N, E, B, A>(v, jt, tw, tw_step, dir);
+  if constexpr (C > 1) {
+    __syncthreads();
+    store_pass<N, E, B, A>(v, lds, jt, addr);
+    __syncthreads();
+    load_pass<N, E, C>(v, lds, jt, addr);
+    bfly_pass<N, E, C, A * B>(v, jt, tw, tw_step, dir);
+  }
+  __syncthreads();
+}
+
+template <int N, class Addr>
+__device__ __forceinline__ void fft_forward_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
+                                                  int tw_step, float dir, Addr addr) {
+  using S = Spec<N>;
+  fft_regs<N, S::E, S::R0, S::R1, S::R2>(v, lds, jt, tw, tw_step, dir, addr);
+}
+
+// Radices reversed: input order = output order of fft_forward_order.
+template <int N, class Addr>
+__device__ __forceinline__ void fft_reverse_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
+                                                  int tw_step, float dir, Addr addr) {
+  using S = Spec<N>;
+  if constexpr (S::R2 > 1)
+    fft_regs<N, S::E, S::R2, S::R1, S::R0>(v, lds, jt, tw, tw_step, dir, addr);
+  else
+    fft_regs<N, S::E, S::R1, S::R0, 1>(v, lds, jt, tw, tw_step, dir, addr);
+}
+
+template <int N>
+constexpr int last_radix() {
+  return Spec<N>::R2 > 1 ? Spec<N>::R2 : Spec<N>::R1;
+}
+
+// ---------------------------------------------------------------------------
+// y pass: complex FFT along y of L consecutive kx columns for one z.
+
+struct YFastParams {
+  float2* spec;
+  const float2* tw;
+  int hx, zlo, ylo, yhi;
+  float dir;
+};
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
+  using S = Spec<N>;
+  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  __shared__ float2 tw[N];
+  __shared__ float2 lds[N * LP];
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
+  const int col = blockIdx.x * L + line;
+  const bool valid = col < P.hx;
+  const int z = P.zlo + blockIdx.y;
+  float2* base = P.spec + (int64_t)z * N * P.hx + col;
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) {
+      const int y = elem_index<N, E, S::R0>(jt, b, t);
+      v[b * S::R0 + t] = valid ? base[(int64_t)y * P.hx] : make_float2(0.f, 0.f);
+    }
+  auto addr = [line](int i) { return i * LP + line; };
+  fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
+  constexpr int RL = last_radix<N>();
+  if (valid) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int y = elem_index<N, E, RL>(jt, b, t);
+        if (y >= P.ylo && y < P.yhi) base[(int64_t)y * P.hx] = v[b * RL + t];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z pass: forward FFT along z, product with the kernel spectrum, inverse FFT,
+// write the z rows [zlo, zhi).  Without a kernel spectrum: forward + scale.
+
+struct ZFastParams {
+  float2* spec;
+  const float2* kspec;
+  const float2* tw;
+  float scale;
+  int hx, qy, zlo, zhi;
+};
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
+  using S = Spec<N>;
+  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  __shared__ float2 tw[N];
+  __shared__ float2 lds[N * LP];
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
+  const int col = blockIdx.x * L + line;
+  const bool valid = col < P.hx;
+  const int64_t plane = (int64_t)P.qy * P.hx;
+  const int64_t off = (int64_t)blockIdx.y * P.hx + col;
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) {
+      const int z = elem_index<N, E, S::R0>(jt, b, t);
+      v[b * S::R0 + t] = valid ? P.spec[z * plane + off] : make_float2(0.f, 0.f);
+    }
+  auto addr = [line](int i) { return i * LP + line; };
+  fft_forward_order<N>(v, lds, jt, tw, 1, -1.f, addr);
+  constexpr int RL = last_radix<N>();
+  if (P.kspec) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int z = elem_index<N, E, RL>(jt, b, t);
+        const float2 k = valid ? P.kspec[z * plane + off] : make_float2(0.f, 0.f);
+        v[b * RL + t] = cmul(v[b * RL + t], k);
+      }
+    fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
+    if (valid) {
+#pragma unroll
+      for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+        for (int t = 0; t < S::R0; ++t) {
+          const int z = elem_index<N, E, S::R0>(jt, b, t);
+          if (z >= P.zlo && z < P.zhi) P.spec[z * plane + off] = v[b * S::R0 + t];
+        }
+    }
+  } else if (valid) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int z = elem_index<N, E, RL>(jt, b, t);
+        P.spec[z * plane + off] = cscale(v[b * RL + t], P.scale);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x passes: rows of 2M real samples <-> M + 1 complex values.  Rows are staged
+// row-major in LDS (coalesced global traffic); the FFT reads them with the
+// rows fastest across lanes.
+
+__device__ __forceinline__ int mirror_fast(int p, int n) {
+  if (p >= 0 && p < n) return p;
+  if (n == 1) return 0;
+  const int period = 2 * n - 2;
+  p %= period;
+  if (p < 0) p += period;
+  return p < n ? p : period - p;
+}
+
+struct XFwdFastParams {
+  ConvGeom g;
+  const float* src;
+  int gather;
+  float2* spec;
+  const float2* tw;  // 2M entries
+  int hx;
+};
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
+  using S = Spec<M>;
+  constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  __shared__ float2 tw[2 * M];
+  __shared__ float2 buf[L * MP];
+  float* fbuf = reinterpret_cast<float*>(buf);
+  const ConvGeom& g = P.g;
+  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
+  const int nrows = g.q.y * g.q.z;
+  const int row0 = blockIdx.x * L;
+  // gather: lanes run along x
+  for (int idx = threadIdx.x; idx < L * 2 * M; idx += kThreads) {
+    const int l = idx / (2 * M), x = idx % (2 * M);
+    const int row = row0 + l;
+    float val = 0.f;
+    if (row < nrows) {
+      const int y = row % g.q.y, z = row / g.q.y;
+      if (x < g.win.x && y < g.win.y && z < g.win.z) {
+        if (P.gather == kGatherKernelWrap) {
+          int jx = x + g.src.x / 2, jy = y + g.src.y / 2, jz = z + g.src.z / 2;
+          if (jx >= g.q.x) jx -= g.q.x;
+          if (jy >= g.q.y) jy -= g.q.y;
+          if (jz >= g.q.z) jz -= g.q.z;
+          if (jx < g.src.x && jy < g.src.y && jz < g.src.z)
+            val = P.src[jx + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz)];
+        } else {
+          const int gx = g.off.x + x, gy = g.off.y + y, gz = g.off.z + z;
+          const bool in_yz = gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
+          const bool in_x = gx >= 0 && gx < g.src.x;
+          if (in_x && in_yz) {
+            val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+          } else if (P.gather == kGatherOne) {
+            val = 1.f;
+          } else if (P.gather == kGatherMirror) {
+            const int mx = mirror_fast(gx, g.src.x), my = mirror_fast(gy, g.src.y), mz = mirror_fast(gz, g.src.z);
+            val = P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
+          }
+        }
+      }
+    }
+    fbuf[l * 2 * MP + x] = val;
+  }
+  __syncthreads();
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  auto addr = [line](int i) { return line * MP + i; };
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) v[b * S::R0 + t] = buf[addr(elem_index<M, E, S::R0>(jt, b, t))];
+  fft_forward_order<M>(v, buf, jt, tw, 2, -1.f, addr);
+  constexpr int RL = last_radix<M>();
+#pragma unroll
+  for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+    for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
+  __syncthreads();
+  // unpack Z = FFT(even + i odd) into X[k] for the pair (k, M - k)
+  constexpr int K2 = M / 2 + 1;
+  for (int idx = threadIdx.x; idx < L * K2; idx += kThreads) {
+    const int l = idx / K2, k = idx % K2;
+    const int row = row0 + l;
+    if (row >= nrows) continue;
+    const float2 a = buf[l * MP + k], b = buf[l * MP + ((M - k) & (M - 1))];
+    float2* out = P.spec + (int64_t)row * P.hx;
+    {
+      const float2 bc = cconj(b);
+      const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
+      out[k] = cadd(E2, cmul(tw[k], make_float2(D.y, -D.x)));
+    }
+    if (M - k != k) {
+      const float2 ac = cconj(a);
+      const float2 E2 = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
+      out[M - k] = cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x)));
+    }
+  }
+}
+
+struct XInvFastParams {
+  ConvGeom g;
+  const float2* spec;
+  EpilogueArgs e;
+  const float2* tw;  // 2M entries
+  int hx;
+};
+
+// Shared with the generic C2R kernel.
+__device__ __forceinline__ float next_value_f(float last, float integral, float weight, double lambda) {
+#pragma clang fp contract(off)
+  const float minv = 0.0001f;
+  const float value = last * integral;
+  float adjusted;
+  if (value > 0.f) {
+    if (lambda > 0.0)
+      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
+    else
+      adjusted = value;
+  } else {
+    adjusted = minv;
+  }
+  const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
+  return last + ((next - last) * weight);
+}
+
+__device__ __forceinline__ float quotient_f(float blurred, float img) {
+#pragma clang fp contract(off)
+  return img > 0.f ? img / blurred : 1.f;
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
+  using S = Spec<M>;
+  constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  __shared__ float2 tw[2 * M];
+  __shared__ float2 buf[L * MP];
+  __shared__ double red_sum[kThreads / 64];
+  __shared__ float red_max[kThreads / 64];
+  const ConvGeom& g = P.g;
+  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
+  const int ny = g.eff_size.y;
+  const int nrows = ny * g.eff_size.z;
+  const int row0 = blockIdx.x * L;
+  for (int idx = threadIdx.x; idx < L * MP; idx += kThreads) {
+    const int l = idx / MP, k = idx % MP;
+    const int row = row0 + l;
+    float2 val = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
+      val = P.spec[((int64_t)z * g.q.y + y) * P.hx + k];
+    }
+    buf[l * MP + k] = val;
+  }
+  __syncthreads();
+  constexpr int K2 = M / 2 + 1;
+  for (int idx = threadIdx.x; idx < L * K2; idx += kThreads) {
+    const int l = idx / K2, k = idx % K2;
+    const float2 a = buf[l * MP + k], b = buf[l * MP + M - k];
+    const float2 za = cadd(cadd(a, cconj(b)), mul_di(cmul(csub(a, cconj(b)), cconj(tw[k])), 1.f));
+    if (k != 0 && M - k != k) {
+      const float2 zb = cadd(cadd(b, cconj(a)), mul_di(cmul(csub(b, cconj(a)), cconj(tw[M - k])), 1.f));
+      buf[l * MP + M - k] = zb;
+    }
+    buf[l * MP + k] = za;
+  }
+  __syncthreads();
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  auto addr = [line](int i) { return line * MP + i; };
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) v[b * S::R0 + t] = buf[addr(elem_index<M, E, S::R0>(jt, b, t))];
+  fft_forward_order<M>(v, buf, jt, tw, 2, 1.f, addr);
+  constexpr int RL = last_radix<M>();
+#pragma unroll
+  for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+    for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
+  __syncthreads();
+  const float* fbuf = reinterpret_cast<const float*>(buf);
+  const EpilogueArgs& e = P.e;
+  double my_sum = 0.0;
+  float my_max = -1.f;
+  const int nx = g.eff_size.x;
+  for (int idx = threadIdx.x; idx < L * nx; idx += kThreads) {
+    const int l = idx / nx, xo = idx - l * nx;
+    const int row = row0 + l;
+    if (row >= nrows) continue;
+    const int x = g.eff_local.x + xo;
+    const float val = fbuf[l * 2 * MP + x];
+    const int yo = row % ny, zo = row / ny;
+    const int64_t gi = (int64_t)(g.eff_offset.x + xo) +
+                       (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
+    if (e.mode == kEpiStore) {
+      e.dst[gi] = val;
+    } else if (e.mode == kEpiQuotient) {
+      e.dst[gi] = quotient_f(val, e.img[gi]);
+    } else {
+      float w = 1.f;
+      if (e.weight_mode == kWeightArray) {
+        w = e.weight[gi];
+      } else if (e.weight_mode == kWeightVirtual) {
+        const double r = (double)e.weight[gi] / (double)e.wsum[gi];
+        w = (float)fmin(1.0, r * e.osem);
+      }
+      const float last = e.dst[gi];
+      const float next = next_value_f(last, val, w, e.lambda);
+      e.dst[gi] = next;
+      const float change = fabsf(next - last);
+      my_sum += (double)change;
+      my_max = fmaxf(my_max, change);
+    }
+  }
+  if (e.mode == kEpiUpdate) {
+    for (int o = 32; o > 0; o >>= 1) {
+      my_sum += __shfl_xor(my_sum, o);
+      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+    }
+    const int wid = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+      red_sum[wid] = my_sum;
+      red_max[wid] = my_max;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      float m = -1.f;
+      for (int i = 0; i < kThreads / 64; ++i) {
+        s += red_sum[i];
+        m = fmaxf(m, red_max[i]);
+      }
+      e.part_sum[blockIdx.x] = s;
+      e.part_max[blockIdx.x] = m;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+
+template <int N>
+static constexpr int lines_per_wg() {
+  return kThreads / (N / Spec<N>::E);
+}
+
+bool fast_size(int n) { return n == 64 || n == 128 || n == 256 || n == 512; }
+
+int x_fast_rows_per_wg(int M) {
+  switch (M) {
+    case 64: return lines_per_wg<64>();
+    case 128: return lines_per_wg<128>();
+    case 256: return lines_per_wg<256>();
+    case 512: return lines_per_wg<512>();
+  }
+  return 0;
+}
+
+hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                         hipStream_t stream) {
+  YFastParams P{spec, sp.tw_y, sp.hx, zlo, ylo, yhi, dir};
+  const int n = sp.q.y;
+  (void)hipGetLastError();
+#define MVD_Y(N)                                                                                \
+  case N: {                                                                                     \
+    dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), zhi - zlo);                  \
+    hipLaunchKernelGGL(k_y_fast<N>, grid, dim3(kThreads), 0, stream, P);                        \
+    break;                                                                                      \
+  }
+  switch (n) {
+    MVD_Y(64)
+    MVD_Y(128)
+    MVD_Y(256)
+    MVD_Y(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef MVD_Y
+  return hipGetLastError();
+}
+
+hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
+                         hipStream_t stream) {
+  ZFastParams P{spec, kspec, sp.tw_z, scale, sp.hx, sp.q.y, zlo, zhi};
+  (void)hipGetLastError();
+#define MVD_Z(N)                                                                                \
+  case N: {                                                                                     \
+    dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), sp.q.y);                     \
+    hipLaunchKernelGGL(k_z_fast<N>, grid, dim3(kThreads), 0, stream, P);                        \
+    break;                                                                                      \
+  }
+  switch (sp.q.z) {
+    MVD_Z(64)
+    MVD_Z(128)
+    MVD_Z(256)
+    MVD_Z(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef MVD_Z
+  return hipGetLastError();
+}
+
+hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                                 hipStream_t stream) {
+  XFwdFastParams P{g, src, gather, spec, sp.tw_x, sp.hx};
+  const int M = g.q.x / 2;
+  const int nrows = g.q.y * g.q.z;
+  (void)hipGetLastError();
+#define MVD_XF(N)                                                                               \
+  case N: {                                                                                     \
+    dim3 grid((nrows + lines_per_wg<N>() - 1) / lines_per_wg<N>());                             \
+    hipLaunchKernelGGL(k_x_forward_fast<N>, grid, dim3(kThreads), 0, stream, P);                \
+    break;                                                                                      \
+  }
+  switch (M) {
+    MVD_XF(64)
+    MVD_XF(128)
+    MVD_XF(256)
+    MVD_XF(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef MVD_XF
+  return hipGetLastError();
+}
+
+hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                                 hipStream_t stream, int nwg) {
+  XInvFastParams P{g, spec, e, sp.tw_x, sp.hx};
+  const int M = g.q.x / 2;
+  (void)hipGetLastError();
+#define MVD_XI(N)                                                                               \
+  case N:                                                                                       \
+    hipLaunchKernelGGL(k_x_inverse_fast<N>, dim3(nwg), dim3(kThreads), 0, stream, P);           \
+    break;
+  switch (M) {
+    MVD_XI(64)
+    MVD_XI(128)
+    MVD_XI(256)
+    MVD_XI(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef MVD_XI
+  return hipGetLastError();
+}
+
+}  // namespace mvd

@@ -105,6 +105,18 @@ hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_
                                    hipStream_t stream);
 hipError_t launch_osem_scale(float* weights, int64_t n, float osem, hipStream_t stream);
 
+// Register-resident kernels for power-of-two lengths (fft_fast.hip).
+bool fast_size(int n);
+int x_fast_rows_per_wg(int M);
+hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                         hipStream_t stream);
+hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
+                         hipStream_t stream);
+hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                                 hipStream_t stream);
+hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                                 hipStream_t stream, int nwg);
+
 // Plan cache and kernel-spectrum helpers (session.cpp).
 void set_error(const std::string& msg);
 

@@ -38,8 +38,12 @@ def test_device_query(jna):
     assert jna.getCUDAcomputeCapabilityMajorVersion(n + 5) == -1
 
 
-@pytest.mark.parametrize("bshape,kshape", [((20, 24, 32), (3, 7, 5)), ((94, 146, 146), (31, 19, 19)),
-                                           ((16, 30, 50), (5, 5, 9)), ((64, 64, 64), (13, 11, 9))])
+@pytest.mark.parametrize("bshape,kshape", [
+    ((20, 24, 32), (3, 7, 5)), ((94, 146, 146), (31, 19, 19)), ((16, 30, 50), (5, 5, 9)),
+    # power-of-two FFT lengths take the register-resident kernels
+    ((64, 64, 64), (13, 11, 9)), ((128, 128, 128), (31, 19, 19)), ((64, 256, 256), (51, 31, 31)),
+    ((64, 64, 512), (9, 9, 41)), ((32, 64, 1024), (5, 7, 9)), ((512, 64, 64), (81, 9, 9)),
+    ((256, 128, 128), (3, 3, 3))])
 def test_jna_inplace_interior_matches_linear_convolution(jna, bshape, kshape):
     rng = np.random.default_rng(sum(bshape))
     blk = rng.random(bshape).astype(np.float32)
@@ -109,6 +113,8 @@ def _small_dataset(seed=7, size=(40, 36, 24), nv=3):
 
 
 @pytest.mark.parametrize("ptype,lam,block", [
+    (PSFTYPE.EFFICIENT_BAYESIAN, 0.006, (64, 64, 64)),
+    (PSFTYPE.EFFICIENT_BAYESIAN, 0.0, (128, 64, 64)),
     (PSFTYPE.EFFICIENT_BAYESIAN, 0.0, None),
     (PSFTYPE.EFFICIENT_BAYESIAN, 0.006, None),
     (PSFTYPE.OPTIMIZATION_I, 0.006, (24, 20, 16)),
