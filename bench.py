@@ -39,7 +39,9 @@ def log(*a):
 
 
 def kernel_bytes(geom_blocks, q, img_n):
-    """Algorithmic HBM bytes of each pass of one view (all blocks), per class."""
+    """Algorithmic HBM bytes of each pass class for one view-iteration (all
+    blocks; the four transform passes run once in each of the two
+    convolutions)."""
     qx, qy, qz = q
     hx = qx // 2 + 1
     H = hx * qy * qz
@@ -48,10 +50,10 @@ def kernel_bytes(geom_blocks, q, img_n):
              x_inverse_update=0.0)
     for (ex, ey, ez) in geom_blocks:
         E = ex * ey * ez
-        b["x_forward"] += 4 * P + 8 * H
-        b["y_forward"] += 16 * H
-        b["z_conv"] += 8 * H + 8 * H + 8 * H * ez / qz
-        b["y_inverse"] += 8 * H * ez / qz + 8 * H * ez * ey / (qz * qy)
+        b["x_forward"] += 2 * (4 * P + 8 * H)
+        b["y_forward"] += 2 * 16 * H
+        b["z_conv"] += 2 * (8 * H + 8 * H + 8 * H * ez / qz)
+        b["y_inverse"] += 2 * (8 * H * ez / qz + 8 * H * ez * ey / (qz * qy))
         rows = hx * ey * ez * 8
         b["x_inverse_quotient"] += rows + 8 * E
         b["x_inverse_update"] += rows + 12 * E

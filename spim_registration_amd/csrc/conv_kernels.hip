@@ -36,7 +36,7 @@ struct XFwdParams {
   float2* spec;
   FftPlan plan;
   const float2* tw;
-  int hx;
+  int hx, pitch;
   int lines;
 };
 
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
     if (row >= nrows) continue;
     const int km = (M - k) % M;
     const float2 a = buf[k * lp + l], b = buf[km * lp + l];
-    float2* out = P.spec + (int64_t)row * P.hx;
+    float2* out = P.spec + (int64_t)row * P.pitch;
     {
       const float2 bc = cconj(b);
       const float2 E = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
 
 struct YParams {
   float2* spec;
-  int hx, qy, zlo, ylo, yhi, lines;
+  int hx, pitch, qy, zlo, ylo, yhi, lines;
   float dir;
   FftPlan plan;
   const float2* tw;
@@ -123,17 +123,17 @@ __global__ __launch_bounds__(kThreads) void k_y(YParams P) {
   for (int i = threadIdx.x; i < qy; i += kThreads) tw[i] = P.tw[i];
   const int c0 = blockIdx.x * B;
   const int z = P.zlo + blockIdx.y;
-  float2* base = P.spec + (int64_t)z * qy * P.hx + c0;
+  float2* base = P.spec + (int64_t)z * qy * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qy * B; idx += kThreads) {
     const int y = idx / B, b = idx - y * B;
-    buf[y * lp + b] = (c0 + b < P.hx) ? base[(int64_t)y * P.hx + b] : make_float2(0.f, 0.f);
+    buf[y * lp + b] = (c0 + b < P.hx) ? base[(int64_t)y * P.pitch + b] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft(buf, lp, B, P.plan, tw, 1, P.dir);
   const int nrow = P.yhi - P.ylo;
   for (int idx = threadIdx.x; idx < nrow * B; idx += kThreads) {
     const int y = P.ylo + idx / B, b = idx % B;
-    if (c0 + b < P.hx) base[(int64_t)y * P.hx + b] = buf[y * lp + b];
+    if (c0 + b < P.hx) base[(int64_t)y * P.pitch + b] = buf[y * lp + b];
   }
 }
 
@@ -141,7 +141,7 @@ struct ZParams {
   float2* spec;
   const float2* kspec;
   float scale;
-  int hx, qy, qz, zlo, zhi, lines;
+  int hx, pitch, qy, qz, zlo, zhi, lines;
   FftPlan plan;
   const float2* tw;
 };
@@ -154,8 +154,8 @@ __global__ __launch_bounds__(kThreads) void k_z(ZParams P) {
   for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = P.tw[i];
   const int c0 = blockIdx.x * B;
   const int y = blockIdx.y;
-  const int64_t plane = (int64_t)P.qy * P.hx;
-  const int64_t col = (int64_t)y * P.hx + c0;
+  const int64_t plane = (int64_t)P.qy * P.pitch;
+  const int64_t col = (int64_t)y * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
     const int z = idx / B, b = idx - z * B;
     buf[z * lp + b] = (c0 + b < P.hx) ? P.spec[z * plane + col + b] : make_float2(0.f, 0.f);
@@ -212,7 +212,7 @@ struct XInvParams {
   EpilogueArgs e;
   FftPlan plan;
   const float2* tw;
-  int hx, lines;
+  int hx, pitch, lines;
 };
 
 __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) {
       const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-      v = P.spec[((int64_t)z * g.q.y + y) * P.hx + k];
+      v = P.spec[((int64_t)z * g.q.y + y) * P.pitch + k];
     }
     buf[k * lp + l] = v;
   }
@@ -326,6 +326,7 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
   P.plan = sp.px;
   P.tw = sp.tw_x;
   P.hx = sp.hx;
+  P.pitch = sp.pitch;
   P.lines = sp.bx;
   const int nrows = g.q.y * g.q.z;
   const int M = g.q.x / 2;
@@ -343,6 +344,7 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
   YParams P;
   P.spec = spec;
   P.hx = sp.hx;
+  P.pitch = sp.pitch;
   P.qy = sp.q.y;
   P.zlo = zlo;
   P.ylo = ylo;
@@ -366,6 +368,7 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
   P.kspec = kspec;
   P.scale = scale;
   P.hx = sp.hx;
+  P.pitch = sp.pitch;
   P.qy = sp.q.y;
   P.qz = sp.q.z;
   P.zlo = zlo;
@@ -395,6 +398,7 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
   P.plan = sp.px;
   P.tw = sp.tw_x;
   P.hx = sp.hx;
+  P.pitch = sp.pitch;
   P.lines = sp.bx;
   const int M = g.q.x / 2;
   const int n = x_inverse_workgroups(sp, g);

@@ -169,7 +169,7 @@ constexpr int last_radix() {
 struct YFastParams {
   float2* spec;
   const float2* tw;
-  int hx, zlo, ylo, yhi;
+  int hx, pitch, zlo, ylo, yhi;
   float dir;
 };
 
@@ -184,14 +184,14 @@ __global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
   const int col = blockIdx.x * L + line;
   const bool valid = col < P.hx;
   const int z = P.zlo + blockIdx.y;
-  float2* base = P.spec + (int64_t)z * N * P.hx + col;
+  float2* base = P.spec + (int64_t)z * N * P.pitch + col;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
     for (int t = 0; t < S::R0; ++t) {
       const int y = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = valid ? base[(int64_t)y * P.hx] : make_float2(0.f, 0.f);
+      v[b * S::R0 + t] = valid ? base[(int64_t)y * P.pitch] : make_float2(0.f, 0.f);
     }
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int y = elem_index<N, E, RL>(jt, b, t);
-        if (y >= P.ylo && y < P.yhi) base[(int64_t)y * P.hx] = v[b * RL + t];
+        if (y >= P.ylo && y < P.yhi) base[(int64_t)y * P.pitch] = v[b * RL + t];
       }
   }
 }
@@ -216,7 +216,7 @@ struct ZFastParams {
   const float2* kspec;
   const float2* tw;
   float scale;
-  int hx, qy, zlo, zhi;
+  int hx, pitch, qy, zlo, zhi;
 };
 
 template <int N>
@@ -229,8 +229,8 @@ __global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
   const int col = blockIdx.x * L + line;
   const bool valid = col < P.hx;
-  const int64_t plane = (int64_t)P.qy * P.hx;
-  const int64_t off = (int64_t)blockIdx.y * P.hx + col;
+  const int64_t plane = (int64_t)P.qy * P.pitch;
+  const int64_t off = (int64_t)blockIdx.y * P.pitch + col;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
@@ -289,16 +289,20 @@ __device__ __forceinline__ int mirror_fast(int p, int n) {
 struct XFwdFastParams {
   ConvGeom g;
   const float* src;
-  int gather;
   float2* spec;
   const float2* tw;  // 2M entries
-  int hx;
+  int pitch;
 };
 
-template <int M>
+// One row of the window: rows are (y, z) of the FFT box, x-fastest.  Loops
+// run over whole rows so that the (y, z) bookkeeping is done once per row.
+template <int M, int MODE>
 __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
   using S = Spec<M>;
   constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  constexpr int W = 2 * M;                                   // real samples per row
+  constexpr int RPI = W >= kThreads ? 1 : kThreads / W;      // rows per loop trip
+  constexpr int EPT = W >= kThreads ? W / kThreads : 1;      // samples per thread per row
   __shared__ float2 tw[2 * M];
   __shared__ float2 buf[L * MP];
   float* fbuf = reinterpret_cast<float*>(buf);
@@ -306,37 +310,57 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
   for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
   const int nrows = g.q.y * g.q.z;
   const int row0 = blockIdx.x * L;
-  // gather: lanes run along x
-  for (int idx = threadIdx.x; idx < L * 2 * M; idx += kThreads) {
-    const int l = idx / (2 * M), x = idx % (2 * M);
+  const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
+  const int lsub = RPI > 1 ? threadIdx.x / W : 0;
+  for (int l0 = 0; l0 < L; l0 += RPI) {
+    const int l = l0 + lsub;
     const int row = row0 + l;
-    float val = 0.f;
-    if (row < nrows) {
-      const int y = row % g.q.y, z = row / g.q.y;
-      if (x < g.win.x && y < g.win.y && z < g.win.z) {
-        if (P.gather == kGatherKernelWrap) {
-          int jx = x + g.src.x / 2, jy = y + g.src.y / 2, jz = z + g.src.z / 2;
+    const int y = row % g.q.y, z = row / g.q.y;
+    const bool live = row < nrows && y < g.win.y && z < g.win.z;
+    const float* srow = nullptr;
+    bool row_inside = false;  // the whole (y, z) row is inside the source
+    if (live) {
+      if (MODE == kGatherKernelWrap) {
+        int jy = y + g.src.y / 2, jz = z + g.src.z / 2;
+        if (jy >= g.q.y) jy -= g.q.y;
+        if (jz >= g.q.z) jz -= g.q.z;
+        row_inside = jy < g.src.y && jz < g.src.z;
+        srow = P.src + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz);
+      } else {
+        int gy = g.off.y + y, gz = g.off.z + z;
+        row_inside = gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
+        if (MODE == kGatherMirror && !row_inside) {
+          gy = mirror_fast(gy, g.src.y);
+          gz = mirror_fast(gz, g.src.z);
+          row_inside = true;  // the mirrored row is a source row
+        }
+        srow = P.src + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int x = x0 + e * kThreads;
+      float val = 0.f;
+      if (live && x < g.win.x) {
+        if (MODE == kGatherKernelWrap) {
+          int jx = x + g.src.x / 2;
           if (jx >= g.q.x) jx -= g.q.x;
-          if (jy >= g.q.y) jy -= g.q.y;
-          if (jz >= g.q.z) jz -= g.q.z;
-          if (jx < g.src.x && jy < g.src.y && jz < g.src.z)
-            val = P.src[jx + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz)];
+          if (row_inside && jx < g.src.x) val = srow[jx];
         } else {
-          const int gx = g.off.x + x, gy = g.off.y + y, gz = g.off.z + z;
-          const bool in_yz = gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
+          int gx = g.off.x + x;
           const bool in_x = gx >= 0 && gx < g.src.x;
-          if (in_x && in_yz) {
-            val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
-          } else if (P.gather == kGatherOne) {
-            val = 1.f;
-          } else if (P.gather == kGatherMirror) {
-            const int mx = mirror_fast(gx, g.src.x), my = mirror_fast(gy, g.src.y), mz = mirror_fast(gz, g.src.z);
-            val = P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
+          if (MODE == kGatherMirror) {
+            if (!in_x) gx = mirror_fast(gx, g.src.x);
+            val = srow[gx];
+          } else if (row_inside && in_x) {
+            val = srow[gx];
+          } else {
+            val = MODE == kGatherOne ? 1.f : 0.f;
           }
         }
       }
+      fbuf[l * 2 * MP + x] = val;
     }
-    fbuf[l * 2 * MP + x] = val;
   }
   __syncthreads();
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
@@ -360,7 +384,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
     const int row = row0 + l;
     if (row >= nrows) continue;
     const float2 a = buf[l * MP + k], b = buf[l * MP + ((M - k) & (M - 1))];
-    float2* out = P.spec + (int64_t)row * P.hx;
+    float2* out = P.spec + (int64_t)row * P.pitch;
     {
       const float2 bc = cconj(b);
       const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
@@ -379,10 +403,9 @@ struct XInvFastParams {
   const float2* spec;
   EpilogueArgs e;
   const float2* tw;  // 2M entries
-  int hx;
+  int pitch;
 };
 
-// Shared with the generic C2R kernel.
 __device__ __forceinline__ float next_value_f(float last, float integral, float weight, double lambda) {
 #pragma clang fp contract(off)
   const float minv = 0.0001f;
@@ -405,10 +428,15 @@ __device__ __forceinline__ float quotient_f(float blurred, float img) {
   return img > 0.f ? img / blurred : 1.f;
 }
 
-template <int M>
+template <int M, int EPI>
 __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   using S = Spec<M>;
   constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  constexpr int W = 2 * M;
+  constexpr int RPI_C = M >= kThreads ? 1 : kThreads / M;   // rows per trip when loading M complex
+  constexpr int EPT_C = M >= kThreads ? M / kThreads : 1;
+  constexpr int RPI = W >= kThreads ? 1 : kThreads / W;     // rows per trip over 2M reals
+  constexpr int EPT = W >= kThreads ? W / kThreads : 1;
   __shared__ float2 tw[2 * M];
   __shared__ float2 buf[L * MP];
   __shared__ double red_sum[kThreads / 64];
@@ -418,15 +446,33 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
   const int row0 = blockIdx.x * L;
-  for (int idx = threadIdx.x; idx < L * MP; idx += kThreads) {
-    const int l = idx / MP, k = idx % MP;
-    const int row = row0 + l;
-    float2 val = make_float2(0.f, 0.f);
-    if (row < nrows) {
-      const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-      val = P.spec[((int64_t)z * g.q.y + y) * P.hx + k];
+  {
+    const int k0 = RPI_C > 1 ? threadIdx.x % M : threadIdx.x;
+    const int lsub = RPI_C > 1 ? threadIdx.x / M : 0;
+    for (int l0 = 0; l0 < L; l0 += RPI_C) {
+      const int l = l0 + lsub;
+      const int row = row0 + l;
+      const float2* srow = nullptr;
+      if (row < nrows) {
+        const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
+        srow = P.spec + ((int64_t)z * g.q.y + y) * P.pitch;
+      }
+#pragma unroll
+      for (int e = 0; e < EPT_C; ++e) {
+        const int k = k0 + e * kThreads;
+        buf[l * MP + k] = srow ? srow[k] : make_float2(0.f, 0.f);
+      }
     }
-    buf[l * MP + k] = val;
+    // the Nyquist column k = M
+    for (int l = threadIdx.x; l < L; l += kThreads) {
+      const int row = row0 + l;
+      float2 val = make_float2(0.f, 0.f);
+      if (row < nrows) {
+        const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
+        val = P.spec[((int64_t)z * g.q.y + y) * P.pitch + M];
+      }
+      buf[l * MP + M] = val;
+    }
   }
   __syncthreads();
   constexpr int K2 = M / 2 + 1;
@@ -460,36 +506,43 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   double my_sum = 0.0;
   float my_max = -1.f;
   const int nx = g.eff_size.x;
-  for (int idx = threadIdx.x; idx < L * nx; idx += kThreads) {
-    const int l = idx / nx, xo = idx - l * nx;
+  const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
+  const int lsub = RPI > 1 ? threadIdx.x / W : 0;
+  for (int l0 = 0; l0 < L; l0 += RPI) {
+    const int l = l0 + lsub;
     const int row = row0 + l;
     if (row >= nrows) continue;
-    const int x = g.eff_local.x + xo;
-    const float val = fbuf[l * 2 * MP + x];
     const int yo = row % ny, zo = row / ny;
-    const int64_t gi = (int64_t)(g.eff_offset.x + xo) +
-                       (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
-    if (e.mode == kEpiStore) {
-      e.dst[gi] = val;
-    } else if (e.mode == kEpiQuotient) {
-      e.dst[gi] = quotient_f(val, e.img[gi]);
-    } else {
-      float w = 1.f;
-      if (e.weight_mode == kWeightArray) {
-        w = e.weight[gi];
-      } else if (e.weight_mode == kWeightVirtual) {
-        const double r = (double)e.weight[gi] / (double)e.wsum[gi];
-        w = (float)fmin(1.0, r * e.osem);
+    const int64_t base = (int64_t)g.eff_offset.x +
+                         (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
+#pragma unroll
+    for (int ee = 0; ee < EPT; ++ee) {
+      const int xo = x0 + ee * kThreads;
+      if (xo >= nx) continue;
+      const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
+      const int64_t gi = base + xo;
+      if (EPI == kEpiStore) {
+        e.dst[gi] = val;
+      } else if (EPI == kEpiQuotient) {
+        e.dst[gi] = quotient_f(val, e.img[gi]);
+      } else {
+        float w = 1.f;
+        if (e.weight_mode == kWeightArray) {
+          w = e.weight[gi];
+        } else if (e.weight_mode == kWeightVirtual) {
+          const double r = (double)e.weight[gi] / (double)e.wsum[gi];
+          w = (float)fmin(1.0, r * e.osem);
+        }
+        const float last = e.dst[gi];
+        const float next = next_value_f(last, val, w, e.lambda);
+        e.dst[gi] = next;
+        const float change = fabsf(next - last);
+        my_sum += (double)change;
+        my_max = fmaxf(my_max, change);
       }
-      const float last = e.dst[gi];
-      const float next = next_value_f(last, val, w, e.lambda);
-      e.dst[gi] = next;
-      const float change = fabsf(next - last);
-      my_sum += (double)change;
-      my_max = fmaxf(my_max, change);
     }
   }
-  if (e.mode == kEpiUpdate) {
+  if (EPI == kEpiUpdate) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
       my_max = fmaxf(my_max, __shfl_xor(my_max, o));
@@ -535,7 +588,7 @@ int x_fast_rows_per_wg(int M) {
 
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                          hipStream_t stream) {
-  YFastParams P{spec, sp.tw_y, sp.hx, zlo, ylo, yhi, dir};
+  YFastParams P{spec, sp.tw_y, sp.hx, sp.pitch, zlo, ylo, yhi, dir};
   const int n = sp.q.y;
   (void)hipGetLastError();
 #define MVD_Y(N)                                                                                \
@@ -557,7 +610,7 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
 
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
                          hipStream_t stream) {
-  ZFastParams P{spec, kspec, sp.tw_z, scale, sp.hx, sp.q.y, zlo, zhi};
+  ZFastParams P{spec, kspec, sp.tw_z, scale, sp.hx, sp.pitch, sp.q.y, zlo, zhi};
   (void)hipGetLastError();
 #define MVD_Z(N)                                                                                \
   case N: {                                                                                     \
@@ -578,14 +631,24 @@ hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec,
 
 hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                                  hipStream_t stream) {
-  XFwdFastParams P{g, src, gather, spec, sp.tw_x, sp.hx};
+  XFwdFastParams P{g, src, spec, sp.tw_x, sp.pitch};
   const int M = g.q.x / 2;
   const int nrows = g.q.y * g.q.z;
   (void)hipGetLastError();
+#define MVD_XF_MODE(N, MODE)                                                                    \
+  case MODE:                                                                                    \
+    hipLaunchKernelGGL((k_x_forward_fast<N, MODE>), grid, dim3(kThreads), 0, stream, P);        \
+    break;
 #define MVD_XF(N)                                                                               \
   case N: {                                                                                     \
     dim3 grid((nrows + lines_per_wg<N>() - 1) / lines_per_wg<N>());                             \
-    hipLaunchKernelGGL(k_x_forward_fast<N>, grid, dim3(kThreads), 0, stream, P);                \
+    switch (gather) {                                                                           \
+      MVD_XF_MODE(N, kGatherMirror)                                                             \
+      MVD_XF_MODE(N, kGatherOne)                                                                \
+      MVD_XF_MODE(N, kGatherZero)                                                               \
+      MVD_XF_MODE(N, kGatherKernelWrap)                                                         \
+      default: return hipErrorInvalidValue;                                                     \
+    }                                                                                           \
     break;                                                                                      \
   }
   switch (M) {
@@ -596,17 +659,27 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
     default: return hipErrorInvalidValue;
   }
 #undef MVD_XF
+#undef MVD_XF_MODE
   return hipGetLastError();
 }
 
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                                  hipStream_t stream, int nwg) {
-  XInvFastParams P{g, spec, e, sp.tw_x, sp.hx};
+  XInvFastParams P{g, spec, e, sp.tw_x, sp.pitch};
   const int M = g.q.x / 2;
   (void)hipGetLastError();
+#define MVD_XI_EPI(N, EPI)                                                                      \
+  case EPI:                                                                                     \
+    hipLaunchKernelGGL((k_x_inverse_fast<N, EPI>), dim3(nwg), dim3(kThreads), 0, stream, P);    \
+    break;
 #define MVD_XI(N)                                                                               \
   case N:                                                                                       \
-    hipLaunchKernelGGL(k_x_inverse_fast<N>, dim3(nwg), dim3(kThreads), 0, stream, P);           \
+    switch (e.mode) {                                                                           \
+      MVD_XI_EPI(N, kEpiStore)                                                                  \
+      MVD_XI_EPI(N, kEpiQuotient)                                                               \
+      MVD_XI_EPI(N, kEpiUpdate)                                                                 \
+      default: return hipErrorInvalidValue;                                                     \
+    }                                                                                           \
     break;
   switch (M) {
     MVD_XI(64)
@@ -616,6 +689,7 @@ hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const f
     default: return hipErrorInvalidValue;
   }
 #undef MVD_XI
+#undef MVD_XI_EPI
   return hipGetLastError();
 }
 

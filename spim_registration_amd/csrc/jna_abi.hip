@@ -96,6 +96,7 @@ const ShapePlan* plan_for(DeviceContext* c, Int3 q) {
   ShapePlan& sp = dp.sp;
   sp.q = q;
   sp.hx = q.x / 2 + 1;
+  sp.pitch = spectrum_pitch(sp.hx);
   sp.px = make_plan(q.x / 2);
   sp.py = make_plan(q.y);
   sp.pz = make_plan(q.z);

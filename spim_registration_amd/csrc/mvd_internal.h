@@ -56,15 +56,17 @@ struct ConvGeom {
 struct ShapePlan {
   Int3 q;
   int hx;                    // q.x / 2 + 1 (half-spectrum row length)
+  int pitch;                 // row stride of the spectrum in complex values (hx rounded up to 16)
   FftPlan px, py, pz;        // plans for lengths q.x/2, q.y, q.z
   float2* tw_x = nullptr;    // q.x entries: exp(-2 pi i m / q.x)
   float2* tw_y = nullptr;    // q.y entries
   float2* tw_z = nullptr;    // q.z entries
   int bx, by, bz;            // lines per workgroup for the x, y, z passes
-  int64_t spectrum_elems() const { return (int64_t)hx * q.y * q.z; }
+  int64_t spectrum_elems() const { return (int64_t)pitch * q.y * q.z; }
 };
 
 FftPlan make_plan(int n);
+inline int spectrum_pitch(int hx) { return (hx + 15) / 16 * 16; }
 bool fft_size_supported(int n);
 int next_fft_size(int n, bool even);
 

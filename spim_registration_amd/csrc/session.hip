@@ -146,6 +146,7 @@ class PlanCache {
     auto sp = std::make_unique<ShapePlan>();
     sp->q = q;
     sp->hx = q.x / 2 + 1;
+    sp->pitch = spectrum_pitch(sp->hx);
     sp->px = make_plan(q.x / 2);
     sp->py = make_plan(q.y);
     sp->pz = make_plan(q.z);
