@@ -312,8 +312,12 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
   const int row0 = blockIdx.x * L;
   const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
   const int lsub = RPI > 1 ? threadIdx.x / W : 0;
-  for (int l0 = 0; l0 < L; l0 += RPI) {
-    const int l = l0 + lsub;
+  constexpr int NR = L / RPI;  // rows handled by each thread
+  // all loads of the thread first (independent, many in flight), then LDS stores
+  float vals[NR][EPT];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int l = r * RPI + lsub;
     const int row = row0 + l;
     const int y = row % g.q.y, z = row / g.q.y;
     const bool live = row < nrows && y < g.win.y && z < g.win.z;
@@ -359,9 +363,13 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
           }
         }
       }
-      fbuf[l * 2 * MP + x] = val;
+      vals[r][e] = val;
     }
   }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) fbuf[(r * RPI + lsub) * 2 * MP + x0 + e * kThreads] = vals[r][e];
   __syncthreads();
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   auto addr = [line](int i) { return line * MP + i; };
@@ -449,20 +457,23 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   {
     const int k0 = RPI_C > 1 ? threadIdx.x % M : threadIdx.x;
     const int lsub = RPI_C > 1 ? threadIdx.x / M : 0;
-    for (int l0 = 0; l0 < L; l0 += RPI_C) {
-      const int l = l0 + lsub;
-      const int row = row0 + l;
+    constexpr int NR = L / RPI_C;
+    float2 vals[NR][EPT_C];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int row = row0 + r * RPI_C + lsub;
       const float2* srow = nullptr;
       if (row < nrows) {
         const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
         srow = P.spec + ((int64_t)z * g.q.y + y) * P.pitch;
       }
 #pragma unroll
-      for (int e = 0; e < EPT_C; ++e) {
-        const int k = k0 + e * kThreads;
-        buf[l * MP + k] = srow ? srow[k] : make_float2(0.f, 0.f);
-      }
+      for (int e = 0; e < EPT_C; ++e) vals[r][e] = srow ? srow[k0 + e * kThreads] : make_float2(0.f, 0.f);
     }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < EPT_C; ++e) buf[(r * RPI_C + lsub) * MP + k0 + e * kThreads] = vals[r][e];
     // the Nyquist column k = M
     for (int l = threadIdx.x; l < L; l += kThreads) {
       const int row = row0 + l;
@@ -508,32 +519,56 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   const int nx = g.eff_size.x;
   const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
   const int lsub = RPI > 1 ? threadIdx.x / W : 0;
-  for (int l0 = 0; l0 < L; l0 += RPI) {
-    const int l = l0 + lsub;
-    const int row = row0 + l;
-    if (row >= nrows) continue;
-    const int yo = row % ny, zo = row / ny;
+  constexpr int NR = L / RPI;
+  // phase 1: destination indices and the inputs of the epilogue, all rows
+  int64_t gidx[NR][EPT];
+  float in0[NR][EPT], in1[NR][EPT];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int row = row0 + r * RPI + lsub;
+    const bool live_row = row < nrows;
+    const int yo = live_row ? row % ny : 0, zo = live_row ? row / ny : 0;
     const int64_t base = (int64_t)g.eff_offset.x +
                          (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
 #pragma unroll
     for (int ee = 0; ee < EPT; ++ee) {
       const int xo = x0 + ee * kThreads;
-      if (xo >= nx) continue;
+      const bool live = live_row && xo < nx;
+      const int64_t gi = live ? base + xo : -1;
+      gidx[r][ee] = gi;
+      in0[r][ee] = 0.f;
+      in1[r][ee] = 1.f;
+      if (live) {
+        if (EPI == kEpiQuotient) {
+          in0[r][ee] = e.img[gi];
+        } else if (EPI == kEpiUpdate) {
+          in0[r][ee] = e.dst[gi];
+          if (e.weight_mode != kWeightNone) in1[r][ee] = e.weight[gi];
+        }
+      }
+    }
+  }
+  // phase 2: compute and store
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int l = r * RPI + lsub;
+#pragma unroll
+    for (int ee = 0; ee < EPT; ++ee) {
+      const int64_t gi = gidx[r][ee];
+      if (gi < 0) continue;
+      const int xo = x0 + ee * kThreads;
       const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
-      const int64_t gi = base + xo;
       if (EPI == kEpiStore) {
         e.dst[gi] = val;
       } else if (EPI == kEpiQuotient) {
-        e.dst[gi] = quotient_f(val, e.img[gi]);
+        e.dst[gi] = quotient_f(val, in0[r][ee]);
       } else {
-        float w = 1.f;
-        if (e.weight_mode == kWeightArray) {
-          w = e.weight[gi];
-        } else if (e.weight_mode == kWeightVirtual) {
-          const double r = (double)e.weight[gi] / (double)e.wsum[gi];
-          w = (float)fmin(1.0, r * e.osem);
+        float w = in1[r][ee];
+        if (e.weight_mode == kWeightVirtual) {
+          const double rr = (double)in1[r][ee] / (double)e.wsum[gi];
+          w = (float)fmin(1.0, rr * e.osem);
         }
-        const float last = e.dst[gi];
+        const float last = in0[r][ee];
         const float next = next_value_f(last, val, w, e.lambda);
         e.dst[gi] = next;
         const float change = fabsf(next - last);
