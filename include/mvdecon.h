@@ -143,6 +143,18 @@ int mvd_get_kernels(mvd_session* s, int view, float* kernel1, float* kernel2);
  * re-run its init (normalisation, kernel2, spectra). */
 int mvd_set_psf(mvd_session* s, int view, const float* psf);
 
+/* Poisson likelihood of the multiview forward model and its gradient (no
+ * counterpart in the reference; its forward model is convolve1,
+ * MVDeconFFT.java:384-470):  b_v = E_mirror(psi) * P_v,
+ *   L = sum_v sum_x w_v (b_v - y_v ln b_v),  r_v = w_v (1 - y_v / b_v),
+ *   dL/dP_v(j) = sum_x r_v(x) E(psi)(x + k/2 - j),  dL/dpsi = sum_v adjoint_v(r_v)
+ * with the adjoint folding the mirror-single extension back.  psi == NULL uses
+ * the session estimate; psfs == NULL (or psfs[v] == NULL) uses the views' PSFs
+ * as given (not normalised).  grad_psi (n voxels) and grad_psf[v] (the view's
+ * kernel box) are optional.  L and the psi gradient accumulate in FP64. */
+int mvd_nll_grad(mvd_session* s, const float* psi, const float* const* psfs, double* nll, float* grad_psi,
+                 float* const* grad_psf);
+
 /* Per-kernel-class GPU timing (HIP events on the session stream).
  * mvd_set_timing(s, 1) enables and resets; mvd_get_timing fills up to cap
  * classes: total ms and launch counts; returns the number of classes.

@@ -536,6 +536,154 @@ void deconvolve(const RunArgs& a, const std::vector<const T*>& imgs, const std::
     if (tmp1[i] == 0) psi[i] = 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Poisson likelihood of the multiview forward model and its gradient (the
+// differentiation extension of the path; the reference has no gradient, the
+// forward model is its convolve1: b_v = E_mirror(psi) * P_v, MVDeconFFT.java:384-470).
+//
+//   L = sum_v sum_x w_v(x) [ b_v(x) - y_v(x) ln b_v(x) ]      (x in the image)
+//   r_v(x)        = w_v(x) (1 - y_v(x) / b_v(x))
+//   dL/dP_v(j)    = sum_x r_v(x) E(psi)(x + c - j)
+//   dL/dpsi(p)    = sum_v sum_x r_v(x) sum_j P_v(j) [m(x + c - j) = p]
+// with m the mirror-single index map and c = k/2.  Voxels with b <= 0 are
+// skipped (contribute nothing).  Sums in double.
+
+// Full-window correlation / adjoint helpers on a padded box of size q >=
+// n + k - 1 (window origin -c).
+template <class T>
+struct Box {
+  int64_t qx, qy, qz;
+  std::unique_ptr<RealFFT3D<T>> fft;
+};
+
+template <class T>
+Box<T> make_box(Dims d, Dims dk, int threads) {
+  Box<T> b;
+  b.qx = next_fast_size(d.x + dk.x - 1, true);
+  b.qy = next_fast_size(d.y + dk.y - 1, false);
+  b.qz = next_fast_size(d.z + dk.z - 1, false);
+  b.fft = std::make_unique<RealFFT3D<T>>((int)b.qx, (int)b.qy, (int)b.qz, threads);
+  return b;
+}
+
+template <class T>
+double nll_grad(const std::vector<const T*>& imgs, const std::vector<const T*>& weights, const T* psi, Dims d,
+                const std::vector<const T*>& psfs, const std::vector<Dims>& kd, T* grad_psi,
+                const std::vector<T*>& grad_psf, int threads) {
+  using C = std::complex<T>;
+  const int64_t n = d.n();
+  std::vector<double> gpsi(n, 0.0);
+  double total = 0.0;
+  for (size_t v = 0; v < imgs.size(); ++v) {
+    const Dims k = kd[v];
+    const int64_t cx = k.x / 2, cy = k.y / 2, cz = k.z / 2;
+    std::vector<T> blur(n);
+    fft_convolve(psi, d, OOB_MIRROR, psfs[v], k, blur.data(), threads);
+    std::vector<T> r(n);
+    double part = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double b = (double)blur[i];
+      const double w = weights.empty() || !weights[v] ? 1.0 : (double)weights[v][i];
+      if (!(b > 0)) {
+        r[i] = 0;
+        continue;
+      }
+      const double y = (double)imgs[v][i];
+      part += w * (b - y * std::log(b));
+      r[i] = (T)(w * (1.0 - y / b));
+    }
+    total += part;
+    // window boxes: W(q) = E(psi)(q - c), R(q) = r(q - c) (zero outside the image)
+    Box<T> box = make_box<T>(d, k, threads);
+    const int64_t qx = box.qx, qy = box.qy, qz = box.qz, q = qx * qy * qz;
+    std::vector<T> W(q, (T)0), R(q, (T)0), H(q, (T)0);
+    for (int64_t z = 0; z < qz; ++z)
+      for (int64_t y = 0; y < qy; ++y)
+        for (int64_t x = 0; x < qx; ++x) {
+          const int64_t i = x + qx * (y + qy * z);
+          if (x < d.x + k.x - 1 && y < d.y + k.y - 1 && z < d.z + k.z - 1)
+            W[i] = extended(psi, d, OOB_MIRROR, x - cx, y - cy, z - cz);
+          const int64_t px = x - cx, py = y - cy, pz = z - cz;
+          if (px >= 0 && px < d.x && py >= 0 && py < d.y && pz >= 0 && pz < d.z)
+            R[i] = r[px + d.x * (py + d.y * pz)];
+        }
+    for (int64_t z = 0; z < k.z; ++z)
+      for (int64_t y = 0; y < k.y; ++y)
+        for (int64_t x = 0; x < k.x; ++x) {
+          const int64_t hx = ((x - cx) % qx + qx) % qx, hy = ((y - cy) % qy + qy) % qy,
+                        hz = ((z - cz) % qz + qz) % qz;
+          H[hx + qx * (hy + qy * hz)] = psfs[v][x + k.x * (y + k.y * z)];
+        }
+    const int64_t hs = box.fft->spectrum_size();
+    std::vector<C> sw(hs), sr(hs), sh(hs), prod(hs);
+    box.fft->forward(W.data(), sw.data());
+    box.fft->forward(R.data(), sr.data());
+    box.fft->forward(H.data(), sh.data());
+    const T scale = (T)(1.0L / (long double)q);
+    // dL/dP(j) = corr(R, W)(c - j),  corr(R, W) = IFFT(conj(R^) W^)
+    for (int64_t i = 0; i < hs; ++i) prod[i] = std::conj(sr[i]) * sw[i];
+    std::vector<T> corr(q);
+    box.fft->inverse(prod.data(), corr.data());
+    for (int64_t z = 0; z < k.z; ++z)
+      for (int64_t y = 0; y < k.y; ++y)
+        for (int64_t x = 0; x < k.x; ++x) {
+          const int64_t sx = ((cx - x) % qx + qx) % qx, sy = ((cy - y) % qy + qy) % qy,
+                        sz = ((cz - z) % qz + qz) % qz;
+          grad_psf[v][x + k.x * (y + k.y * z)] = corr[sx + qx * (sy + qy * sz)] * scale;
+        }
+    // adjoint on the window: G(q) = IFFT(R^ conj(H^)); fold window position q
+    // (image position q - c, mirror-extended) back onto the image
+    for (int64_t i = 0; i < hs; ++i) prod[i] = sr[i] * std::conj(sh[i]);
+    box.fft->inverse(prod.data(), corr.data());
+    for (int64_t z = 0; z < d.z + k.z - 1; ++z)
+      for (int64_t y = 0; y < d.y + k.y - 1; ++y)
+        for (int64_t x = 0; x < d.x + k.x - 1; ++x) {
+          const int64_t px = mirror_index(x - cx, d.x), py = mirror_index(y - cy, d.y),
+                        pz = mirror_index(z - cz, d.z);
+          gpsi[px + d.x * (py + d.y * pz)] += (double)(corr[x + qx * (y + qy * z)] * scale);
+        }
+  }
+  for (int64_t i = 0; i < n; ++i) grad_psi[i] = (T)gpsi[i];
+  return total;
+}
+
+// L for one view in double with the blur given (b = blur + h * shift_j psi):
+// central differences of single PSF voxels, each an O(n) rank-one update.
+double nll_view_fd(const double* img, const double* weight, const double* blur, const double* psi, Dims d,
+                   Dims k, const int64_t* voxels, int nvox, double h, double* out, int threads) {
+  const int64_t cx = k.x / 2, cy = k.y / 2, cz = k.z / 2;
+  for (int t = 0; t < nvox; ++t) {
+    const int64_t j = voxels[t];
+    const int64_t jx = j % k.x, jy = (j / k.x) % k.y, jz = j / (k.x * k.y);
+    // per-z-slice partial sums, each slice written by one thread, summed in order
+    std::vector<double> sp(d.z, 0.0), sm(d.z, 0.0);
+    parallel_for(d.z, threads, [&](int64_t zb, int64_t ze) {
+      for (int64_t z = zb; z < ze; ++z) {
+        double ap = 0, am = 0;
+        for (int64_t y = 0; y < d.y; ++y)
+          for (int64_t x = 0; x < d.x; ++x) {
+            const int64_t i = x + d.x * (y + d.y * z);
+            const double s = extended(psi, d, OOB_MIRROR, x + cx - jx, y + cy - jy, z + cz - jz);
+            const double w = weight ? weight[i] : 1.0;
+            const double bp = blur[i] + h * s, bm = blur[i] - h * s;
+            if (bp > 0) ap += w * (bp - img[i] * std::log(bp));
+            if (bm > 0) am += w * (bm - img[i] * std::log(bm));
+          }
+        sp[z] = ap;
+        sm[z] = am;
+      }
+    });
+    double lp = 0, lm = 0;
+    for (int64_t z = 0; z < d.z; ++z) {
+      lp += sp[z];
+      lm += sm[z];
+    }
+    out[t] = (lp - lm) / (2 * h);
+  }
+  return 0;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -673,6 +821,30 @@ int orc_direct_convolve_f64(const double* in, const int64_t* dims, int oob, cons
         for (int v = 0; v < nviews; ++v) std::copy(k2[v].begin(), k2[v].end(), k2_out[v]);             \
     })                                                                                                 \
   }
+
+
+int orc_nll_view_fd_f64(const double* img, const double* weight, const double* blur, const double* psi,
+                        const int64_t* dims, const int64_t* kdims, const int64_t* voxels, int nvox, double h,
+                        double* out, int threads) {
+  ORC_TRY(nll_view_fd(img, weight, blur, psi, {dims[0], dims[1], dims[2]}, {kdims[0], kdims[1], kdims[2]}, voxels,
+                      nvox, h, out, threads))
+}
+
+#define ORC_GRAD(T, SUF)                                                                               \
+  int orc_nll_grad_##SUF(int nviews, const int64_t* dims, const T* const* imgs, const T* const* weights,  \
+                         const T* psi, const T* const* psfs, const int64_t* kdims, T* grad_psi,          \
+                         T* const* grad_psf, double* nll, int threads) {                                 \
+    ORC_TRY({                                                                                          \
+      std::vector<const T*> im(imgs, imgs + nviews), w, ps(psfs, psfs + nviews);                       \
+      if (weights) w.assign(weights, weights + nviews);                                                \
+      std::vector<Dims> kd;                                                                            \
+      for (int v = 0; v < nviews; ++v) kd.push_back({kdims[3 * v], kdims[3 * v + 1], kdims[3 * v + 2]}); \
+      std::vector<T*> gp(grad_psf, grad_psf + nviews);                                                 \
+      *nll = nll_grad(im, w, psi, {dims[0], dims[1], dims[2]}, ps, kd, grad_psi, gp, threads);          \
+    })                                                                                                 \
+  }
+ORC_GRAD(float, f32)
+ORC_GRAD(double, f64)
 
 ORC_TYPED(float, f32)
 ORC_TYPED(double, f64)

@@ -240,3 +240,50 @@ def deconvolve(views, kernels, psf_type: int, iterations: int, lam: float, weigh
     if return_kernels:
         return psi, stats, k2
     return psi, stats
+
+
+def _declare_grad(L):
+    L.orc_nll_view_fd_f64.argtypes = [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_double, _P, ctypes.c_int]
+    for suf in ("f32", "f64"):
+        getattr(L, f"orc_nll_grad_{suf}").argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                      ctypes.c_int]
+
+
+def nll_grad(views, psfs, psi, weights=None, threads=None):
+    """Poisson NLL of the forward model b_v = E_mirror(psi) * P_v and its
+    gradient.  Returns (nll, grad_psi, [grad_psf_v])."""
+    L = lib()
+    _declare_grad(L)
+    dtype = np.dtype(psi.dtype)
+    views = [np.ascontiguousarray(v, dtype=dtype) for v in views]
+    psfs = [np.ascontiguousarray(p, dtype=dtype) for p in psfs]
+    psi = np.ascontiguousarray(psi)
+    w = None
+    if weights is not None:
+        weights = [np.ascontiguousarray(x, dtype=dtype) for x in weights]
+        w = _ptr_array(weights)
+    gpsi = np.empty_like(psi)
+    gpsf = [np.empty_like(p) for p in psfs]
+    kd = np.concatenate([_dims(p) for p in psfs]).astype(np.int64)
+    nll = ctypes.c_double()
+    _check(getattr(L, f"orc_nll_grad_{_dt(dtype)}")(
+        len(views), _ptr(_dims(psi)), _ptr_array(views), w, _ptr(psi), _ptr_array(psfs), _ptr(kd), _ptr(gpsi),
+        _ptr_array(gpsf), ctypes.byref(nll), threads or default_threads()))
+    return nll.value, gpsi, gpsf
+
+
+def nll_psf_fd(view, psf, psi, voxels, h=1e-3, weight=None, threads=None):
+    """Central finite differences of one view's NLL w.r.t. single PSF voxels
+    (flat indices), in FP64: each probe is a rank-one update of the blur."""
+    L = lib()
+    _declare_grad(L)
+    v = np.ascontiguousarray(view, dtype=np.float64)
+    p = np.ascontiguousarray(psf, dtype=np.float64)
+    s = np.ascontiguousarray(psi, dtype=np.float64)
+    blur = fft_convolve(s, p, OOB_MIRROR, threads)
+    wt = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+    vox = np.ascontiguousarray(voxels, dtype=np.int64)
+    out = np.empty(len(vox), np.float64)
+    _check(L.orc_nll_view_fd_f64(_ptr(v), None if wt is None else _ptr(wt), _ptr(blur), _ptr(s), _ptr(_dims(s)),
+                                 _ptr(_dims(p)), _ptr(vox), len(vox), h, _ptr(out), threads or default_threads()))
+    return out

@@ -21,7 +21,7 @@ EXPORTS = (
     "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
     "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
-    "mvd_timing_name", "mvd_get_geometry", "mvd_last_error",
+    "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad",
 )
 
 
@@ -61,6 +61,7 @@ def _declare(L):
     L.mvd_timing_name.restype = ctypes.c_char_p
     L.mvd_get_geometry.argtypes = [_P, ctypes.c_int, _P]
     L.mvd_last_error.restype = ctypes.c_char_p
+    L.mvd_nll_grad.argtypes = [_P, _P, _P, _P, _P, _P]
 
 
 def lib():

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void k_y(YParams P) {
 struct ZParams {
   float2* spec;
   const float2* kspec;
-  float scale;
+  float scale, dir;
   int hx, pitch, qy, qz, zlo, zhi, lines;
   FftPlan plan;
   const float2* tw;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_z(ZParams P) {
     buf[z * lp + b] = (c0 + b < P.hx) ? P.spec[z * plane + col + b] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(buf, lp, B, P.plan, tw, 1, -1.f);
+  lds_fft(buf, lp, B, P.plan, tw, 1, P.kspec ? -1.f : P.dir);
   if (P.kspec) {
     for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
       const int z = idx / B, b = idx - z * B;
@@ -270,6 +270,15 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
                        (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
     if (e.mode == kEpiStore) {
       e.dst[gi] = v;
+    } else if (e.mode == kEpiResidual) {
+      const double w = e.weight_mode == kWeightArray ? (double)e.weight[gi] : 1.0;
+      const double b = (double)v, y = (double)e.img[gi];
+      if (b > 0) {
+        my_sum += w * (b - y * log(b));
+        e.dst[gi] = (float)(w * (1.0 - y / b));
+      } else {
+        e.dst[gi] = 0.f;
+      }
     } else if (e.mode == kEpiQuotient) {
       e.dst[gi] = quotient(v, e.img[gi]);
     } else {
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
       my_max = fmaxf(my_max, change);
     }
   }
-  if (e.mode == kEpiUpdate) {
+  if (e.mode == kEpiUpdate || e.mode == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
       my_max = fmaxf(my_max, __shfl_xor(my_max, o));
@@ -361,9 +370,10 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
 }
 
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream) {
-  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream);
+                    hipStream_t stream, float dir) {
+  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir);
   ZParams P;
+  P.dir = dir;
   P.spec = spec;
   P.kspec = kspec;
   P.scale = scale;

@@ -215,7 +215,7 @@ struct ZFastParams {
   float2* spec;
   const float2* kspec;
   const float2* tw;
-  float scale;
+  float scale, dir;
   int hx, pitch, qy, zlo, zhi;
 };
 
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
       v[b * S::R0 + t] = valid ? P.spec[z * plane + off] : make_float2(0.f, 0.f);
     }
   auto addr = [line](int i) { return i * LP + line; };
-  fft_forward_order<N>(v, lds, jt, tw, 1, -1.f, addr);
+  fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
   constexpr int RL = last_radix<N>();
   if (P.kspec) {
 #pragma unroll
@@ -539,8 +539,9 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
       in0[r][ee] = 0.f;
       in1[r][ee] = 1.f;
       if (live) {
-        if (EPI == kEpiQuotient) {
+        if (EPI == kEpiQuotient || EPI == kEpiResidual) {
           in0[r][ee] = e.img[gi];
+          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[r][ee] = e.weight[gi];
         } else if (EPI == kEpiUpdate) {
           in0[r][ee] = e.dst[gi];
           if (e.weight_mode != kWeightNone) in1[r][ee] = e.weight[gi];
@@ -562,6 +563,14 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
         e.dst[gi] = val;
       } else if (EPI == kEpiQuotient) {
         e.dst[gi] = quotient_f(val, in0[r][ee]);
+      } else if (EPI == kEpiResidual) {
+        const double w = (double)in1[r][ee], b = (double)val, y = (double)in0[r][ee];
+        if (b > 0) {
+          my_sum += w * (b - y * log(b));
+          e.dst[gi] = (float)(w * (1.0 - y / b));
+        } else {
+          e.dst[gi] = 0.f;
+        }
       } else {
         float w = in1[r][ee];
         if (e.weight_mode == kWeightVirtual) {
@@ -577,7 +586,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
       }
     }
   }
-  if (EPI == kEpiUpdate) {
+  if (EPI == kEpiUpdate || EPI == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
       my_max = fmaxf(my_max, __shfl_xor(my_max, o));
@@ -644,8 +653,8 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
 }
 
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream) {
-  ZFastParams P{spec, kspec, sp.tw_z, scale, sp.hx, sp.pitch, sp.q.y, zlo, zhi};
+                         hipStream_t stream, float dir) {
+  ZFastParams P{spec, kspec, sp.tw_z, scale, dir, sp.hx, sp.pitch, sp.q.y, zlo, zhi};
   (void)hipGetLastError();
 #define MVD_Z(N)                                                                                \
   case N: {                                                                                     \
@@ -713,6 +722,7 @@ hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const f
       MVD_XI_EPI(N, kEpiStore)                                                                  \
       MVD_XI_EPI(N, kEpiQuotient)                                                               \
       MVD_XI_EPI(N, kEpiUpdate)                                                                 \
+      MVD_XI_EPI(N, kEpiResidual)                                                               \
       default: return hipErrorInvalidValue;                                                     \
     }                                                                                           \
     break;

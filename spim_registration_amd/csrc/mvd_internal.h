@@ -24,6 +24,7 @@ enum Epilogue : int {
   kEpiStore = 0,     // dst = result
   kEpiQuotient = 1,  // dst = img > 0 ? img / result : 1   (MVDeconvolution.java:494-546)
   kEpiUpdate = 2,    // psi update + statistics          (MVDeconvolution.java:603-724)
+  kEpiResidual = 3,  // dst = w (1 - img / result), FP64 partials of w (result - img ln result)
 };
 
 // Weight source used by the update epilogue.
@@ -88,9 +89,10 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
                             hipStream_t stream);
 hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                     hipStream_t stream);
-// kspec == nullptr: forward z transform then multiply by `scale` (kernel spectra).
+// kspec == nullptr: one z transform in direction `dir`, then multiply by `scale`
+// (kernel spectra, gradient passes).
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream);
+                    hipStream_t stream, float dir = -1.f);
 // Returns the number of workgroups (for the update statistics) in *nwg.
 hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                             hipStream_t stream, int* nwg);
@@ -113,11 +115,22 @@ int x_fast_rows_per_wg(int M);
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                          hipStream_t stream);
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream);
+                         hipStream_t stream, float dir);
 hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                                  hipStream_t stream);
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                                  hipStream_t stream, int nwg);
+
+// Gradient helpers (gradient.hip).
+// out = op(a) * op(b) over n spectrum values, op = conjugate when the flag is set.
+hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, int conj_a, int conj_b, int64_t n,
+                               hipStream_t stream);
+// grad[j] = scale * box[(c - j) mod q] for the kernel box kd (x-fastest).
+hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, hipStream_t stream);
+// acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
+hipError_t launch_fold_adjoint(const float* win, Int3 wdim, Int3 n, Int3 c, double* acc, hipStream_t stream);
+hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
+hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream);
 
 // Plan cache and kernel-spectrum helpers (session.cpp).
 void set_error(const std::string& msg);

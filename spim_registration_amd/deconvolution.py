@@ -250,6 +250,29 @@ class MVDeconvolution:
         _native.check(_native.lib().mvd_get_geometry(self._s.handle, view, g), "mvd_get_geometry")
         return dict(blocks=g[0], fft=(g[1], g[2], g[3]))
 
+    def nll_grad(self, psi=None, psfs=None, want_psi: bool = True, want_psf: bool = True):
+        """Poisson NLL of the forward model (b_v = E_mirror(psi) * P_v) and its
+        gradient w.r.t. the estimate and each view's PSF (mvd_nll_grad).
+        psi / psfs default to the current estimate and the views' PSFs.
+        Returns (nll, grad_psi or None, [grad_psf_v] or None)."""
+        L = _native.lib()
+        s = self._s
+        p0 = None if psi is None else _f32(psi)
+        kp = None
+        keep = []
+        if psfs is not None:
+            ks = [_f32(k) for k in psfs]
+            keep += ks
+            kp = (ctypes.c_void_p * len(ks))(*[k.ctypes.data for k in ks])
+        gpsi = np.empty(s.shape, np.float32) if want_psi else None
+        gpsf = [np.empty_like(v.kernel1) for v in s.views] if want_psf else None
+        gp = None
+        if gpsf is not None:
+            gp = (ctypes.c_void_p * len(gpsf))(*[g.ctypes.data for g in gpsf])
+        nll = ctypes.c_double()
+        _native.check(L.mvd_nll_grad(s.handle, _ptr(p0), kp, ctypes.byref(nll), _ptr(gpsi), gp), "mvd_nll_grad")
+        return nll.value, gpsi, gpsf
+
     def set_timing(self, on: bool):
         _native.check(_native.lib().mvd_set_timing(self._s.handle, 1 if on else 0), "mvd_set_timing")
 
