@@ -95,6 +95,13 @@ enum {
   MVD_WEIGHTS_VIRTUAL = 3      /* w_v / max(sum, 1) evaluated per voxel (NormalizingRandomAccess) */
 };
 
+/* FFT shape of each block window (call before mvd_prepare). */
+enum {
+  MVD_FFT_SMOOTH = 0,   /* smallest 5-smooth size per axis (default) */
+  MVD_FFT_POW2 = 1,     /* smallest power of two per axis (up to 1024) */
+  MVD_FFT_AUTOTUNE = 2  /* time both candidates per axis on the GPU, keep the fastest (cached per shape) */
+};
+
 /* new MVDeconInput + image geometry: nviews views of nx*ny*nz voxels on `device`. */
 int mvd_create(int device, int nx, int ny, int nz, int nviews, mvd_session** out);
 int mvd_destroy(mvd_session* s);
@@ -114,6 +121,10 @@ int mvd_add_view(mvd_session* s, const float* img, const float* weight, const fl
  * min_avg_out (nullable) receives {min overlapping views, avg overlapping views}. */
 int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value, int norm_threads,
                     double* min_avg_out);
+
+/* Choose the FFT shape policy (MVD_FFT_*); no counterpart in the reference,
+ * whose FFT sizes are chosen by its FFT library. */
+int mvd_set_fft_policy(mvd_session* s, int policy);
 
 /* MVDeconInput.init(PSFTYPE) (deconv/MVDeconInput.java:62-68 -> MVDeconFFT.init,
  * deconv/MVDeconFFT.java:183-324) plus block split.  block_xyz NULL = the whole
@@ -166,6 +177,23 @@ const char* mvd_timing_name(int cls);
 /* Geometry summary after mvd_prepare for one view: number of blocks and the
  * FFT size used; out[0..3] = {nblocks, qx, qy, qz}. */
 int mvd_get_geometry(mvd_session* s, int view, int* out);
+
+/* ---- Batched iteration over many stacks (no counterpart in the reference) --
+ * A batch runs the iterations of several prepared sessions (independent
+ * stacks of any size and view count) together: in view slot v every stack
+ * with more than v views is updated, each FFT pass is one launch per kernel
+ * class for all (stack, block) items, and with use_graph != 0 a whole
+ * iteration is captured once and replayed as a hipGraph.  Per stack the
+ * result is identical to mvd_iterate.  Sessions are borrowed: they must stay
+ * alive and must not be iterated on their own while in a prepared batch.
+ * stats (nullable): n iterations x (sum over stacks of nviews) x {sum, max},
+ * stacks in the order they were added. */
+typedef struct mvd_batch mvd_batch;
+int mvd_batch_create(int device, mvd_batch** out);
+int mvd_batch_add(mvd_batch* b, mvd_session* s);
+int mvd_batch_prepare(mvd_batch* b, int use_graph);
+int mvd_batch_iterate(mvd_batch* b, int n, double* stats);
+int mvd_batch_destroy(mvd_batch* b);
 
 /* Thread-local message of the last failed call. */
 const char* mvd_last_error(void);

@@ -16,6 +16,8 @@
 //               epilogue (store, quotient or multiplicative update).
 // Kernel spectra are pre-scaled by 1/(q.x q.y q.z), so every transform here
 // is unnormalised.
+#include <cstring>
+
 #include "mvd_internal.h"
 
 namespace mvd {
@@ -61,7 +63,7 @@ __device__ __forceinline__ float gather_value(const XFwdParams& P, int x, int y,
   return P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
 }
 
-__global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
+__device__ __forceinline__ void x_forward_body(const XFwdParams& P, int bx, int by) {
   extern __shared__ float2 smem[];
   const int qx = P.g.q.x, M = qx / 2;
   const int B = P.lines, lp = B + 1;
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
   float2* buf = smem + qx;
   for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
   const int nrows = P.g.q.y * P.g.q.z;
-  const int row0 = blockIdx.x * B;
+  const int row0 = bx * B;
   float* fbuf = reinterpret_cast<float*>(buf);
   for (int idx = threadIdx.x; idx < B * qx; idx += kThreads) {
     const int l = idx / qx, x = idx - l * qx;
@@ -107,6 +109,15 @@ __global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_x_forward(XFwdParams P) { x_forward_body(P, blockIdx.x, blockIdx.y); }
+
+__global__ __launch_bounds__(kThreads) void k_x_forward_batch(const XFwdParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  x_forward_body(items[item], bx, by);
+}
+
 struct YParams {
   float2* spec;
   int hx, pitch, qy, zlo, ylo, yhi, lines;
@@ -115,14 +126,14 @@ struct YParams {
   const float2* tw;
 };
 
-__global__ __launch_bounds__(kThreads) void k_y(YParams P) {
+__device__ __forceinline__ void y_body(const YParams& P, int bx, int by) {
   extern __shared__ float2 smem[];
   const int B = P.lines, lp = B + 1, qy = P.qy;
   float2* tw = smem;
   float2* buf = smem + qy;
   for (int i = threadIdx.x; i < qy; i += kThreads) tw[i] = P.tw[i];
-  const int c0 = blockIdx.x * B;
-  const int z = P.zlo + blockIdx.y;
+  const int c0 = bx * B;
+  const int z = P.zlo + by;
   float2* base = P.spec + (int64_t)z * qy * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qy * B; idx += kThreads) {
     const int y = idx / B, b = idx - y * B;
@@ -137,6 +148,15 @@ __global__ __launch_bounds__(kThreads) void k_y(YParams P) {
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_y(YParams P) { y_body(P, blockIdx.x, blockIdx.y); }
+
+__global__ __launch_bounds__(kThreads) void k_y_batch(const YParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  y_body(items[item], bx, by);
+}
+
 struct ZParams {
   float2* spec;
   const float2* kspec;
@@ -146,14 +166,14 @@ struct ZParams {
   const float2* tw;
 };
 
-__global__ __launch_bounds__(kThreads) void k_z(ZParams P) {
+__device__ __forceinline__ void z_body(const ZParams& P, int bx, int by) {
   extern __shared__ float2 smem[];
   const int B = P.lines, lp = B + 1, qz = P.qz;
   float2* tw = smem;
   float2* buf = smem + qz;
   for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = P.tw[i];
-  const int c0 = blockIdx.x * B;
-  const int y = blockIdx.y;
+  const int c0 = bx * B;
+  const int y = by;
   const int64_t plane = (int64_t)P.qy * P.pitch;
   const int64_t col = (int64_t)y * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
@@ -180,6 +200,15 @@ __global__ __launch_bounds__(kThreads) void k_z(ZParams P) {
       if (c0 + b < P.hx) P.spec[z * plane + col + b] = cscale(buf[z * lp + b], P.scale);
     }
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_z(ZParams P) { z_body(P, blockIdx.x, blockIdx.y); }
+
+__global__ __launch_bounds__(kThreads) void k_z_batch(const ZParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  z_body(items[item], bx, by);
 }
 
 // MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726),
@@ -215,7 +244,7 @@ struct XInvParams {
   int hx, pitch, lines;
 };
 
-__global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
+__device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int by) {
   extern __shared__ float2 smem[];
   __shared__ double red_sum[kThreads / 64];
   __shared__ float red_max[kThreads / 64];
@@ -227,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
   for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
-  const int row0 = blockIdx.x * B;
+  const int row0 = bx * B;
   const int H = M + 1;
   for (int idx = threadIdx.x; idx < B * H; idx += kThreads) {
     const int l = idx / H, k = idx - l * H;
@@ -315,10 +344,19 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) {
         s += red_sum[i];
         m = fmaxf(m, red_max[i]);
       }
-      e.part_sum[blockIdx.x] = s;
-      e.part_max[blockIdx.x] = m;
+      e.part_sum[bx] = s;
+      e.part_max[bx] = m;
     }
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_x_inverse(XInvParams P) { x_inverse_body(P, blockIdx.x, blockIdx.y); }
+
+__global__ __launch_bounds__(kThreads) void k_x_inverse_batch(const XInvParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  x_inverse_body(items[item], bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -418,6 +456,120 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
   size_t lds = sizeof(float2) * (g.q.x + (size_t)(M + 1) * (P.lines + 1));
   (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_x_inverse, dim3(n), dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched generic passes
+
+int build_generic_params(const PassDesc& d, void* out, int* gx, int* gy, size_t* lds) {
+  const ShapePlan& sp = *d.sp;
+  const ConvGeom& g = d.g;
+  switch (d.kind) {
+    case kPassXForward: {
+      XFwdParams P{};
+      P.g = g;
+      P.src = d.src;
+      P.gather = d.gather;
+      P.spec = d.spec;
+      P.plan = sp.px;
+      P.tw = sp.tw_x;
+      P.hx = sp.hx;
+      P.pitch = sp.pitch;
+      P.lines = sp.bx;
+      *gx = (g.q.y * g.q.z + P.lines - 1) / P.lines;
+      *gy = 1;
+      *lds = sizeof(float2) * (g.q.x + (size_t)(g.q.x / 2) * (P.lines + 1));
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassYForward:
+    case kPassYInverse: {
+      YParams P{};
+      P.spec = d.spec;
+      P.hx = sp.hx;
+      P.pitch = sp.pitch;
+      P.qy = sp.q.y;
+      P.zlo = d.zlo;
+      P.ylo = d.ylo;
+      P.yhi = d.yhi;
+      P.lines = sp.by;
+      P.dir = d.dir;
+      P.plan = sp.py;
+      P.tw = sp.tw_y;
+      *gx = (sp.hx + P.lines - 1) / P.lines;
+      *gy = d.zhi - d.zlo;
+      *lds = sizeof(float2) * (sp.q.y + (size_t)sp.q.y * (P.lines + 1));
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassZConv: {
+      ZParams P{};
+      P.spec = d.spec;
+      P.kspec = d.kspec;
+      P.scale = d.scale;
+      P.dir = d.dir;
+      P.hx = sp.hx;
+      P.pitch = sp.pitch;
+      P.qy = sp.q.y;
+      P.qz = sp.q.z;
+      P.zlo = d.zlo;
+      P.zhi = d.zhi;
+      P.lines = sp.bz;
+      P.plan = sp.pz;
+      P.tw = sp.tw_z;
+      *gx = (sp.hx + P.lines - 1) / P.lines;
+      *gy = sp.q.y;
+      *lds = sizeof(float2) * (sp.q.z + (size_t)sp.q.z * (P.lines + 1));
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassXInverse: {
+      XInvParams P{};
+      P.g = g;
+      P.spec = d.spec;
+      P.e = d.e;
+      P.plan = sp.px;
+      P.tw = sp.tw_x;
+      P.hx = sp.hx;
+      P.pitch = sp.pitch;
+      P.lines = sp.bx;
+      *gx = x_inverse_workgroups(sp, g);
+      *gy = 1;
+      *lds = sizeof(float2) * (g.q.x + (size_t)(g.q.x / 2 + 1) * (P.lines + 1));
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+  }
+  return -1;
+}
+
+hipError_t launch_batch_generic(int kind, int epi, const void* items, const int* prefix, const int* gx, int nitems,
+                                int nwg, size_t lds, hipStream_t stream) {
+  (void)epi;
+  if (nwg <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  switch (kind) {
+    case kPassXForward:
+      hipLaunchKernelGGL(k_x_forward_batch, dim3(nwg), dim3(kThreads), lds, stream,
+                         static_cast<const XFwdParams*>(items), prefix, gx, nitems);
+      break;
+    case kPassYForward:
+    case kPassYInverse:
+      hipLaunchKernelGGL(k_y_batch, dim3(nwg), dim3(kThreads), lds, stream, static_cast<const YParams*>(items),
+                         prefix, gx, nitems);
+      break;
+    case kPassZConv:
+      hipLaunchKernelGGL(k_z_batch, dim3(nwg), dim3(kThreads), lds, stream, static_cast<const ZParams*>(items),
+                         prefix, gx, nitems);
+      break;
+    case kPassXInverse:
+      hipLaunchKernelGGL(k_x_inverse_batch, dim3(nwg), dim3(kThreads), lds, stream,
+                         static_cast<const XInvParams*>(items), prefix, gx, nitems);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -14,6 +14,8 @@
 // thread holds X[jt + T b + t N/R_last], which is exactly the input order of a
 // transform with the radices reversed -- the fused z pass goes forward,
 // multiplies by the kernel spectrum and back without touching LDS in between.
+#include <cstring>
+
 #include "mvd_internal.h"
 
 namespace mvd {
@@ -174,16 +176,16 @@ struct YFastParams {
 };
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
+__device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
-  const int col = blockIdx.x * L + line;
+  const int col = bx * L + line;
   const bool valid = col < P.hx;
-  const int z = P.zlo + blockIdx.y;
+  const int z = P.zlo + by;
   float2* base = P.spec + (int64_t)z * N * P.pitch + col;
   float2 v[E];
 #pragma unroll
@@ -207,6 +209,19 @@ __global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
+  y_fast_body<N>(P, blockIdx.x, blockIdx.y);
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_y_fast_batch(const YFastParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  y_fast_body<N>(items[item], bx, by);
+}
+
 // ---------------------------------------------------------------------------
 // z pass: forward FFT along z, product with the kernel spectrum, inverse FFT,
 // write the z rows [zlo, zhi).  Without a kernel spectrum: forward + scale.
@@ -220,17 +235,17 @@ struct ZFastParams {
 };
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
+__device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
-  const int col = blockIdx.x * L + line;
+  const int col = bx * L + line;
   const bool valid = col < P.hx;
   const int64_t plane = (int64_t)P.qy * P.pitch;
-  const int64_t off = (int64_t)blockIdx.y * P.pitch + col;
+  const int64_t off = (int64_t)by * P.pitch + col;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
@@ -272,6 +287,19 @@ __global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
+  z_fast_body<N>(P, blockIdx.x, blockIdx.y);
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_z_fast_batch(const ZFastParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  z_fast_body<N>(items[item], bx, by);
+}
+
 // ---------------------------------------------------------------------------
 // x passes: rows of 2M real samples <-> M + 1 complex values.  Rows are staged
 // row-major in LDS (coalesced global traffic); the FFT reads them with the
@@ -297,7 +325,7 @@ struct XFwdFastParams {
 // One row of the window: rows are (y, z) of the FFT box, x-fastest.  Loops
 // run over whole rows so that the (y, z) bookkeeping is done once per row.
 template <int M, int MODE>
-__global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
+__device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int bx, int by) {
   using S = Spec<M>;
   constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
   constexpr int W = 2 * M;                                   // real samples per row
@@ -309,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
   const ConvGeom& g = P.g;
   for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
   const int nrows = g.q.y * g.q.z;
-  const int row0 = blockIdx.x * L;
+  const int row0 = bx * L;
   const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
   const int lsub = RPI > 1 ? threadIdx.x / W : 0;
   constexpr int NR = L / RPI;  // rows handled by each thread
@@ -406,6 +434,19 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
   }
 }
 
+template <int M, int MODE>
+__global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
+  x_forward_fast_body<M, MODE>(P, blockIdx.x, blockIdx.y);
+}
+
+template <int M, int MODE>
+__global__ __launch_bounds__(kThreads) void k_x_forward_fast_batch(const XFwdFastParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  x_forward_fast_body<M, MODE>(items[item], bx, by);
+}
+
 struct XInvFastParams {
   ConvGeom g;
   const float2* spec;
@@ -437,7 +478,7 @@ __device__ __forceinline__ float quotient_f(float blurred, float img) {
 }
 
 template <int M, int EPI>
-__global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
+__device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int bx, int by) {
   using S = Spec<M>;
   constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
   constexpr int W = 2 * M;
@@ -453,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
   for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
-  const int row0 = blockIdx.x * L;
+  const int row0 = bx * L;
   {
     const int k0 = RPI_C > 1 ? threadIdx.x % M : threadIdx.x;
     const int lsub = RPI_C > 1 ? threadIdx.x / M : 0;
@@ -604,10 +645,23 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
         s += red_sum[i];
         m = fmaxf(m, red_max[i]);
       }
-      e.part_sum[blockIdx.x] = s;
-      e.part_max[blockIdx.x] = m;
+      e.part_sum[bx] = s;
+      e.part_max[bx] = m;
     }
   }
+}
+
+template <int M, int EPI>
+__global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
+  x_inverse_fast_body<M, EPI>(P, blockIdx.x, blockIdx.y);
+}
+
+template <int M, int EPI>
+__global__ __launch_bounds__(kThreads) void k_x_inverse_fast_batch(const XInvFastParams* items, const int* prefix, const int* gx,
+                                                   int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  x_inverse_fast_body<M, EPI>(items[item], bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,6 +682,148 @@ int x_fast_rows_per_wg(int M) {
     case 512: return lines_per_wg<512>();
   }
   return 0;
+}
+
+bool pass_is_fast(const PassDesc& d) {
+  switch (d.kind) {
+    case kPassXForward:
+    case kPassXInverse: return fast_size(d.g.q.x / 2);
+    case kPassYForward:
+    case kPassYInverse: return fast_size(d.sp->q.y);
+    case kPassZConv: return fast_size(d.sp->q.z);
+  }
+  return false;
+}
+
+template <int N>
+static int lines_of() {
+  return lines_per_wg<N>();
+}
+
+static int fast_lines(int n) {
+  switch (n) {
+    case 64: return lines_of<64>();
+    case 128: return lines_of<128>();
+    case 256: return lines_of<256>();
+    case 512: return lines_of<512>();
+  }
+  return 1;
+}
+
+int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) {
+  const ShapePlan& sp = *d.sp;
+  const ConvGeom& g = d.g;
+  switch (d.kind) {
+    case kPassXForward: {
+      XFwdFastParams P{g, d.src, d.spec, sp.tw_x, sp.pitch};
+      const int M = g.q.x / 2, L = fast_lines(M);
+      *gx = (g.q.y * g.q.z + L - 1) / L;
+      *gy = 1;
+      *cls = M * 16 + d.gather;
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassYForward:
+    case kPassYInverse: {
+      YFastParams P{d.spec, sp.tw_y, sp.hx, sp.pitch, d.zlo, d.ylo, d.yhi, d.dir};
+      const int L = fast_lines(sp.q.y);
+      *gx = (sp.hx + L - 1) / L;
+      *gy = d.zhi - d.zlo;
+      *cls = sp.q.y;
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassZConv: {
+      ZFastParams P{d.spec, d.kspec, sp.tw_z, d.scale, d.dir, sp.hx, sp.pitch, sp.q.y, d.zlo, d.zhi};
+      const int L = fast_lines(sp.q.z);
+      *gx = (sp.hx + L - 1) / L;
+      *gy = sp.q.y;
+      *cls = sp.q.z;
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+    case kPassXInverse: {
+      XInvFastParams P{g, d.spec, d.e, sp.tw_x, sp.pitch};
+      const int M = g.q.x / 2;
+      *gx = x_inverse_workgroups(sp, g);
+      *gy = 1;
+      *cls = M * 16 + d.e.mode;
+      std::memcpy(out, &P, sizeof(P));
+      return sizeof(P);
+    }
+  }
+  return -1;
+}
+
+hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
+                             int nwg, hipStream_t stream) {
+  if (nwg <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  const dim3 grid(nwg), block(kThreads);
+  switch (kind) {
+    case kPassYForward:
+    case kPassYInverse: {
+      const YFastParams* it = static_cast<const YFastParams*>(items);
+      switch (cls) {
+        case 64: hipLaunchKernelGGL(k_y_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 128: hipLaunchKernelGGL(k_y_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 256: hipLaunchKernelGGL(k_y_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 512: hipLaunchKernelGGL(k_y_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    case kPassZConv: {
+      const ZFastParams* it = static_cast<const ZFastParams*>(items);
+      switch (cls) {
+        case 64: hipLaunchKernelGGL(k_z_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 128: hipLaunchKernelGGL(k_z_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 256: hipLaunchKernelGGL(k_z_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 512: hipLaunchKernelGGL(k_z_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    case kPassXForward: {
+      const XFwdFastParams* it = static_cast<const XFwdFastParams*>(items);
+#define MVD_BXF(M, MODE)                                                                                   \
+  case M * 16 + MODE:                                                                                      \
+    hipLaunchKernelGGL((k_x_forward_fast_batch<M, MODE>), grid, block, 0, stream, it, prefix, gx, nitems); \
+    break;
+#define MVD_BXF4(M) MVD_BXF(M, kGatherMirror) MVD_BXF(M, kGatherOne) MVD_BXF(M, kGatherZero) MVD_BXF(M, kGatherKernelWrap)
+      switch (cls) {
+        MVD_BXF4(64)
+        MVD_BXF4(128)
+        MVD_BXF4(256)
+        MVD_BXF4(512)
+        default: return hipErrorInvalidValue;
+      }
+#undef MVD_BXF4
+#undef MVD_BXF
+      break;
+    }
+    case kPassXInverse: {
+      const XInvFastParams* it = static_cast<const XInvFastParams*>(items);
+#define MVD_BXI(M, EPI)                                                                                    \
+  case M * 16 + EPI:                                                                                       \
+    hipLaunchKernelGGL((k_x_inverse_fast_batch<M, EPI>), grid, block, 0, stream, it, prefix, gx, nitems);  \
+    break;
+#define MVD_BXI4(M) MVD_BXI(M, kEpiStore) MVD_BXI(M, kEpiQuotient) MVD_BXI(M, kEpiUpdate) MVD_BXI(M, kEpiResidual)
+      switch (cls) {
+        MVD_BXI4(64)
+        MVD_BXI4(128)
+        MVD_BXI4(256)
+        MVD_BXI4(512)
+        default: return hipErrorInvalidValue;
+      }
+#undef MVD_BXI4
+#undef MVD_BXI
+      break;
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,

@@ -66,6 +66,23 @@ struct ShapePlan {
   int64_t spectrum_elems() const { return (int64_t)pitch * q.y * q.z; }
 };
 
+// Batched launches: workgroup blockIdx.x belongs to the item i with
+// prefix[i] <= blockIdx.x < prefix[i + 1]; within the item it is workgroup
+// (local % gx[i], local / gx[i]) of that item's own grid.
+__device__ __forceinline__ void batch_locate(const int* prefix, const int* gx, int nitems, int& item, int& bx,
+                                             int& by) {
+  const int wg = blockIdx.x;
+  int lo = 0, hi = nitems;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= wg) lo = mid; else hi = mid;
+  }
+  const int local = wg - prefix[lo];
+  item = lo;
+  bx = local % gx[lo];
+  by = local / gx[lo];
+}
+
 FftPlan make_plan(int n);
 inline int spectrum_pitch(int hx) { return (hx + 15) / 16 * 16; }
 bool fft_size_supported(int n);
@@ -120,6 +137,40 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
                                  hipStream_t stream);
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                                  hipStream_t stream, int nwg);
+
+// Batched passes (batch.hip): one launch runs the same pass for many items
+// (blocks of many stacks), each item with its own geometry and buffers.
+enum PassKind : int { kPassXForward = 0, kPassYForward, kPassZConv, kPassYInverse, kPassXInverse };
+struct PassDesc {
+  int kind;
+  const ShapePlan* sp;
+  ConvGeom g;
+  const float* src;     // x forward source
+  int gather;           // x forward out-of-bounds rule
+  float2* spec;         // block spectrum
+  const float2* kspec;  // z pass kernel spectrum
+  EpilogueArgs e;       // x inverse epilogue
+  int zlo, zhi, ylo, yhi;
+  float dir, scale;
+};
+constexpr int kMaxParamBytes = 640;
+bool pass_is_fast(const PassDesc& d);
+// Fill the kernel parameter block of one item; returns its size in bytes and
+// the item's own grid (gx, gy); *lds = dynamic LDS bytes (generic) and
+// *cls = template selector (fast).
+int build_generic_params(const PassDesc& d, void* out, int* gx, int* gy, size_t* lds);
+int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls);
+hipError_t launch_batch_generic(int kind, int epi, const void* items, const int* prefix, const int* gx, int nitems,
+                                int nwg, size_t lds, hipStream_t stream);
+hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
+                             int nwg, hipStream_t stream);
+struct ReduceItem {
+  const double* part_sum;
+  const float* part_max;
+  int n;
+  double* out;
+};
+hipError_t launch_reduce_stats_batch(const ReduceItem* items, int nitems, hipStream_t stream);
 
 // Gradient helpers (gradient.hip).
 // out = op(a) * op(b) over n spectrum values, op = conjugate when the flag is set.

@@ -124,6 +124,41 @@ hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, in
   return hipGetLastError();
 }
 
+// One workgroup per item: the statistics of one (stack, view) of a batched
+// iteration, reduced in the same order as k_reduce_stats.
+__global__ __launch_bounds__(kPwThreads) void k_reduce_stats_batch(const ReduceItem* items, int nitems) {
+  __shared__ double rs[kPwThreads];
+  __shared__ float rm[kPwThreads];
+  const ReduceItem it = items[blockIdx.x];
+  double s = 0.0;
+  float m = -1.f;
+  for (int i = threadIdx.x; i < it.n; i += kPwThreads) {
+    s += it.part_sum[i];
+    m = fmaxf(m, it.part_max[i]);
+  }
+  rs[threadIdx.x] = s;
+  rm[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = kPwThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      rs[threadIdx.x] += rs[threadIdx.x + o];
+      rm[threadIdx.x] = fmaxf(rm[threadIdx.x], rm[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    it.out[0] = rs[0];
+    it.out[1] = (double)rm[0];
+  }
+}
+
+hipError_t launch_reduce_stats_batch(const ReduceItem* items, int nitems, hipStream_t stream) {
+  if (nitems <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_reduce_stats_batch, dim3(nitems), dim3(kPwThreads), 0, stream, items, nitems);
+  return hipGetLastError();
+}
+
 // WeightNormalizer (deconv/WeightNormalizer.java:138-253).  Per voxel the
 // double sum of the view weights and the number of views with weight > 0.
 // Precomputed weights (ApplyDirectly, :138-185): w_v = (float)(w_v / sum),

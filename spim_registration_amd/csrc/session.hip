@@ -402,6 +402,7 @@ struct mvd_session {
   PlanCache plans;
   Timer timer;
 
+  int fft_policy = MVD_FFT_SMOOTH;
   int psf_type = MVD_PSF_EFFICIENT_BAYESIAN;
   double lambda = 0.0;
   int norm_threads = 8;
@@ -562,11 +563,115 @@ int init_view(mvd_session* s, int v) {
   return 0;
 }
 
+// FFT shape of a block window.  MVD_FFT_SMOOTH: smallest 5-smooth size per
+// axis (even along x); MVD_FFT_POW2: smallest power of two (up to 1024, else
+// 5-smooth); MVD_FFT_AUTOTUNE: both candidates per axis are timed on the GPU
+// (one full block convolution each, best of three) and the fastest shape is
+// kept, cached per window shape for the process.
+static int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, Int3> g_tune_cache;
+
+int time_shape(mvd_session* s, Int3 win, Int3 q, float* ms_out) {
+  const ShapePlan* sp = s->plans.get(q);
+  if (!sp) return -1;
+  DeviceBuf spec, kspec, src;
+  MVD_CHECK(spec.ensure(sizeof(float2) * sp->spectrum_elems()));
+  MVD_CHECK(kspec.ensure(sizeof(float2) * sp->spectrum_elems()));
+  MVD_CHECK(src.ensure(sizeof(float) * (size_t)win.x * win.y * win.z));
+  MVD_CHECK(hipMemsetAsync(kspec.p, 0, sizeof(float2) * sp->spectrum_elems(), s->stream));
+  MVD_CHECK(hipMemsetAsync(src.p, 0, sizeof(float) * (size_t)win.x * win.y * win.z, s->stream));
+  ConvGeom g{};
+  g.src = win;
+  g.off = {0, 0, 0};
+  g.win = win;
+  g.q = q;
+  g.eff_local = {0, 0, 0};
+  g.eff_size = win;
+  g.eff_offset = {0, 0, 0};
+  g.dst = win;
+  EpilogueArgs e{};
+  e.mode = kEpiStore;
+  e.dst = src.as<float>();
+  hipEvent_t a, b;
+  MVD_CHECK(hipEventCreate(&a));
+  MVD_CHECK(hipEventCreate(&b));
+  float best = 1e30f;
+  const bool timing = s->timer.on();
+  s->timer.enable(false);
+  for (int rep = 0; rep < 3; ++rep) {
+    MVD_CHECK(hipEventRecord(a, s->stream));
+    int rc = run_conv(s, *sp, g, src.as<float>(), kGatherZero, kspec.as<float2>(), e, spec.as<float2>());
+    if (rc) return rc;
+    MVD_CHECK(hipEventRecord(b, s->stream));
+    MVD_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    MVD_CHECK(hipEventElapsedTime(&ms, a, b));
+    best = std::min(best, ms);
+  }
+  s->timer.enable(timing);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_out = best;
+  return 0;
+}
+
+int choose_fft_shape(mvd_session* s, Int3 win, Int3* q) {
+  const Int3 smooth = fft_shape_for(win);
+  if (s->fft_policy == MVD_FFT_SMOOTH) {
+    *q = smooth;
+    return 0;
+  }
+  auto p2 = [](int w, int sm) { const int p = pow2_at_least(w); return p <= 1024 ? p : sm; };
+  const Int3 pow2 = {p2(win.x, smooth.x), p2(win.y, smooth.y), p2(win.z, smooth.z)};
+  if (s->fft_policy == MVD_FFT_POW2) {
+    *q = pow2;
+    return 0;
+  }
+  const auto key = std::make_tuple(win.x, win.y, win.z);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *q = it->second;
+      return 0;
+    }
+  }
+  Int3 best = smooth;
+  float best_ms = 1e30f;
+  for (int c = 0; c < 8; ++c) {
+    const Int3 cand = {(c & 1) ? pow2.x : smooth.x, (c & 2) ? pow2.y : smooth.y, (c & 4) ? pow2.z : smooth.z};
+    if (c && ((c & 1) && pow2.x == smooth.x)) continue;
+    if (c && ((c & 2) && pow2.y == smooth.y)) continue;
+    if (c && ((c & 4) && pow2.z == smooth.z)) continue;
+    float ms = 0.f;
+    int rc = time_shape(s, win, cand, &ms);
+    if (rc) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cand;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  *q = best;
+  return 0;
+}
+
 int build_geometry(mvd_session* s, int v) {
   mvd_session::View& view = s->views[v];
   Int3 bs = s->use_blocks ? s->block
                           : Int3{s->img.x + view.kd.x - 1, s->img.y + view.kd.y - 1, s->img.z + view.kd.z - 1};
-  const Int3 q = fft_shape_for(bs);
+  Int3 q;
+  int rc0 = choose_fft_shape(s, bs, &q);
+  if (rc0) return rc0;
   view.plan = s->plans.get(q);
   if (!view.plan) return -1;
   MVD_REQUIRE(divide_into_blocks(s->img, view.kd, bs, q, &view.blocks),
@@ -786,6 +891,14 @@ int mvd_prepare(mvd_session* s, int psf_type, double lambda, const int* block_xy
   int rc = ensure_workspace(s);
   if (rc) return rc;
   s->prepared = true;
+  return 0;
+}
+
+int mvd_set_fft_policy(mvd_session* s, int policy) {
+  MVD_REQUIRE(s, "NULL session");
+  MVD_REQUIRE(policy >= MVD_FFT_SMOOTH && policy <= MVD_FFT_AUTOTUNE, "invalid FFT shape policy");
+  s->fft_policy = policy;
+  s->prepared = false;
   return 0;
 }
 
@@ -1085,6 +1198,301 @@ int mvd_get_timing(mvd_session* s, double* total_ms, long long* launches, int ca
 const char* mvd_timing_name(int cls) {
   if (cls < 0 || cls >= kTNumClasses) return "";
   return kTimingNames[cls];
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched iteration over many sessions (stacks): in view slot v every stack
+// with more than v views runs convolve1 + quotient, then convolve2 + update,
+// for all of its blocks; each pass is one launch per kernel class for all
+// (stack, block) items, and a whole iteration can be replayed as a hipGraph.
+
+struct mvd_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<mvd_session*> stacks;
+  std::vector<std::vector<DeviceBuf>> spec;  // [stack][block]
+  struct Launch {
+    int kind = 0, cls = 0, epi = 0, nitems = 0, nwg = 0;
+    bool fast = false, reduce = false;
+    size_t lds = 0;
+    DeviceBuf items, prefix, gx;
+  };
+  std::vector<Launch> launches;
+  DeviceBuf stats_iter, stats_hist;
+  std::vector<int> stats_off;  // per stack, in doubles
+  int total_stats = 0;
+  bool prepared = false, use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~mvd_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int add_pass_launches(mvd_batch* b, const std::vector<PassDesc>& descs) {
+  // group by kernel class
+  std::map<std::pair<int, int>, std::vector<const PassDesc*>> groups;  // (fast ? cls : -1, epi) -> items
+  for (auto& d : descs) {
+    int cls = -1;
+    if (pass_is_fast(d)) {
+      unsigned char tmp[kMaxParamBytes];
+      int gx, gy;
+      build_fast_params(d, tmp, &gx, &gy, &cls);
+    }
+    groups[{cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
+  }
+  for (auto& kv : groups) {
+    mvd_batch::Launch L;
+    L.kind = kv.second.front()->kind;
+    L.fast = kv.first.first >= 0;
+    L.cls = kv.first.first;
+    L.epi = kv.first.second;
+    std::vector<unsigned char> blob;
+    std::vector<int> prefix(1, 0), gxs;
+    int psize = 0;
+    for (const PassDesc* d : kv.second) {
+      unsigned char tmp[kMaxParamBytes];
+      int gx = 0, gy = 0, cls = 0;
+      size_t lds = 0;
+      const int n = L.fast ? build_fast_params(*d, tmp, &gx, &gy, &cls) : build_generic_params(*d, tmp, &gx, &gy, &lds);
+      MVD_REQUIRE(n > 0 && n <= kMaxParamBytes, "internal: parameter block size");
+      psize = n;
+      blob.insert(blob.end(), tmp, tmp + n);
+      gxs.push_back(std::max(gx, 1));
+      prefix.push_back(prefix.back() + gx * gy);
+      L.lds = std::max(L.lds, lds);
+    }
+    (void)psize;
+    L.nitems = (int)kv.second.size();
+    L.nwg = prefix.back();
+    MVD_CHECK(L.items.ensure(blob.size()));
+    MVD_CHECK(hipMemcpy(L.items.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    MVD_CHECK(L.prefix.ensure(sizeof(int) * prefix.size()));
+    MVD_CHECK(hipMemcpy(L.prefix.p, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice));
+    MVD_CHECK(L.gx.ensure(sizeof(int) * gxs.size()));
+    MVD_CHECK(hipMemcpy(L.gx.p, gxs.data(), sizeof(int) * gxs.size(), hipMemcpyHostToDevice));
+    b->launches.push_back(std::move(L));
+  }
+  return 0;
+}
+
+int enqueue_batch_iteration(mvd_batch* b) {
+  for (auto& L : b->launches) {
+    if (L.reduce) {
+      MVD_CHECK(launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, b->stream));
+    } else if (L.fast) {
+      MVD_CHECK(launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
+                                  b->stream));
+    } else {
+      MVD_CHECK(launch_batch_generic(L.kind, L.epi, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
+                                     L.lds, b->stream));
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvd_batch_create(int device, mvd_batch** out) {
+  MVD_REQUIRE(out, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  MVD_CHECK(hipGetDeviceCount(&ndev));
+  MVD_REQUIRE(device >= 0 && device < ndev, "invalid device id");
+  DeviceGuard guard(device);
+  auto b = std::make_unique<mvd_batch>();
+  b->device = device;
+  MVD_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  *out = b.release();
+  return 0;
+}
+
+int mvd_batch_destroy(mvd_batch* b) {
+  if (!b) return 0;
+  DeviceGuard guard(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return 0;
+}
+
+int mvd_batch_add(mvd_batch* b, mvd_session* s) {
+  MVD_REQUIRE(b && s, "NULL argument");
+  MVD_REQUIRE(s->device == b->device, "session is on another device");
+  MVD_REQUIRE(s->prepared, "mvd_prepare the session first");
+  for (auto* x : b->stacks) MVD_REQUIRE(x != s, "session already in the batch");
+  b->stacks.push_back(s);
+  b->prepared = false;
+  return 0;
+}
+
+int mvd_batch_prepare(mvd_batch* b, int use_graph) {
+  MVD_REQUIRE(b, "NULL batch");
+  MVD_REQUIRE(!b->stacks.empty(), "no sessions in the batch");
+  DeviceGuard guard(b->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->launches.clear();
+  b->spec.clear();
+  b->stats_off.clear();
+  int maxv = 0, tot = 0;
+  for (auto* s : b->stacks) {
+    MVD_REQUIRE(s->prepared, "a session is not prepared");
+    b->stats_off.push_back(tot);
+    tot += 2 * s->nviews;
+    maxv = std::max(maxv, s->nviews);
+    size_t nb = 0;
+    for (auto& v : s->views) nb = std::max(nb, v.blocks.size());
+    b->spec.emplace_back();
+    for (size_t i = 0; i < nb; ++i) {
+      b->spec.back().emplace_back();
+      size_t elems = 0;
+      for (auto& v : s->views)
+        if (i < v.blocks.size()) elems = std::max<size_t>(elems, v.plan->spectrum_elems());
+      MVD_CHECK(b->spec.back().back().ensure(sizeof(float2) * elems));
+    }
+  }
+  b->total_stats = tot;
+  MVD_CHECK(b->stats_iter.ensure(sizeof(double) * tot));
+  for (int v = 0; v < maxv; ++v) {
+    for (int conv = 0; conv < 2; ++conv) {
+      std::vector<PassDesc> pass[5];
+      for (size_t si = 0; si < b->stacks.size(); ++si) {
+        mvd_session* s = b->stacks[si];
+        if (v >= s->nviews) continue;
+        auto& view = s->views[v];
+        const int wm = s->weight_mode == MVD_WEIGHTS_NONE ? kWeightNone
+                       : s->weight_mode == MVD_WEIGHTS_VIRTUAL ? kWeightVirtual
+                                                               : kWeightArray;
+        int off = 0;
+        for (size_t bi = 0; bi < view.blocks.size(); ++bi) {
+          const ConvGeom& g = view.blocks[bi];
+          PassDesc d{};
+          d.sp = view.plan;
+          d.g = g;
+          d.spec = b->spec[si][bi].as<float2>();
+          d.src = conv == 0 ? s->d_psi.as<float>() : s->d_ratio.as<float>();
+          d.gather = conv == 0 ? kGatherMirror : kGatherOne;
+          d.kspec = conv == 0 ? view.k1hat.as<float2>() : view.k2hat.as<float2>();
+          d.scale = 1.f;
+          PassDesc x = d;
+          x.kind = kPassXForward;
+          pass[0].push_back(x);
+          PassDesc yf = d;
+          yf.kind = kPassYForward;
+          yf.zlo = 0;
+          yf.zhi = view.plan->q.z;
+          yf.ylo = 0;
+          yf.yhi = view.plan->q.y;
+          yf.dir = -1.f;
+          pass[1].push_back(yf);
+          PassDesc z = d;
+          z.kind = kPassZConv;
+          z.zlo = g.eff_local.z;
+          z.zhi = g.eff_local.z + g.eff_size.z;
+          z.dir = -1.f;
+          pass[2].push_back(z);
+          PassDesc yi = d;
+          yi.kind = kPassYInverse;
+          yi.zlo = g.eff_local.z;
+          yi.zhi = g.eff_local.z + g.eff_size.z;
+          yi.ylo = g.eff_local.y;
+          yi.yhi = g.eff_local.y + g.eff_size.y;
+          yi.dir = 1.f;
+          pass[3].push_back(yi);
+          PassDesc xi = d;
+          xi.kind = kPassXInverse;
+          if (conv == 0) {
+            xi.e.mode = kEpiQuotient;
+            xi.e.dst = s->d_ratio.as<float>();
+            xi.e.img = s->d_views.as<float>() + v * s->n;
+          } else {
+            xi.e.mode = kEpiUpdate;
+            xi.e.dst = s->d_psi.as<float>();
+            xi.e.weight_mode = wm;
+            xi.e.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
+            xi.e.wsum = s->d_wsum.as<float>();
+            xi.e.osem = s->osem;
+            xi.e.lambda = s->lambda;
+            xi.e.part_sum = s->d_part_sum.as<double>() + off;
+            xi.e.part_max = s->d_part_max.as<float>() + off;
+            off += x_inverse_workgroups(*view.plan, g);
+          }
+          pass[4].push_back(xi);
+        }
+      }
+      for (int p = 0; p < 5; ++p) {
+        int rc = add_pass_launches(b, pass[p]);
+        if (rc) return rc;
+      }
+    }
+    // statistics of view slot v
+    std::vector<ReduceItem> red;
+    for (size_t si = 0; si < b->stacks.size(); ++si) {
+      mvd_session* s = b->stacks[si];
+      if (v >= s->nviews) continue;
+      int n = 0;
+      for (auto& g : s->views[v].blocks) n += x_inverse_workgroups(*s->views[v].plan, g);
+      red.push_back({s->d_part_sum.as<double>(), s->d_part_max.as<float>(), n,
+                     b->stats_iter.as<double>() + b->stats_off[si] + 2 * v});
+    }
+    mvd_batch::Launch L;
+    L.reduce = true;
+    L.nitems = (int)red.size();
+    MVD_CHECK(L.items.ensure(sizeof(ReduceItem) * red.size()));
+    MVD_CHECK(hipMemcpy(L.items.p, red.data(), sizeof(ReduceItem) * red.size(), hipMemcpyHostToDevice));
+    b->launches.push_back(std::move(L));
+  }
+  b->use_graph = use_graph != 0;
+  if (b->use_graph) {
+    MVD_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch_iteration(b);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    MVD_CHECK(ec);
+    b->graph = g;
+    MVD_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  b->prepared = true;
+  return 0;
+}
+
+int mvd_batch_iterate(mvd_batch* b, int n, double* stats) {
+  MVD_REQUIRE(b, "NULL batch");
+  MVD_REQUIRE(b->prepared, "mvd_batch_prepare first");
+  MVD_REQUIRE(n >= 0, "negative iteration count");
+  for (auto* s : b->stacks) MVD_REQUIRE(s->psi_ready, "mvd_init_psi every session first");
+  if (n == 0) return 0;
+  DeviceGuard guard(b->device);
+  MVD_CHECK(b->stats_hist.ensure(sizeof(double) * (size_t)b->total_stats * n));
+  for (int it = 0; it < n; ++it) {
+    if (b->use_graph) {
+      MVD_CHECK(hipGraphLaunch(b->exec, b->stream));
+    } else {
+      int rc = enqueue_batch_iteration(b);
+      if (rc) return rc;
+    }
+    MVD_CHECK(hipMemcpyAsync(b->stats_hist.as<double>() + (size_t)it * b->total_stats, b->stats_iter.p,
+                             sizeof(double) * b->total_stats, hipMemcpyDeviceToDevice, b->stream));
+  }
+  if (stats)
+    MVD_CHECK(hipMemcpyAsync(stats, b->stats_hist.p, sizeof(double) * (size_t)b->total_stats * n,
+                             hipMemcpyDeviceToHost, b->stream));
+  MVD_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -178,7 +178,7 @@ class MVDeconvolution:
     def __init__(self, views: MVDeconInput, iteration_type: PSFTYPE, num_iterations: int, lam: float,
                  osem_speedup: float = 1.0, osem_speedup_index: int = 0, name: str = "deconvolved",
                  norm_threads: int = 8, weight_type: Optional[WeightType] = None,
-                 initial_psi=None, run: bool = True):
+                 initial_psi=None, run: bool = True, fft_policy: int = 0):
         vs = views.get_views()
         if not vs:
             raise ValueError("no views")
@@ -204,6 +204,7 @@ class MVDeconvolution:
         _native.check(L.mvd_set_weights(self._s.handle, int(self.weight_type), int(osem_speedup_index),
                                         float(osem_speedup), int(norm_threads), mm), "mvd_set_weights")
         self.min_overlapping_views, self.avg_overlapping_views = mm[0], mm[1]
+        _native.check(L.mvd_set_fft_policy(self._s.handle, int(fft_policy)), "mvd_set_fft_policy")
         blk = None
         if v0.use_blocks:
             blk = (ctypes.c_int * 3)(*v0.block_size)
@@ -285,3 +286,55 @@ class MVDeconvolution:
 
     def close(self):
         self._s.close()
+
+
+class FFTPolicy(enum.IntEnum):
+    """FFT shape of block windows (mvd_set_fft_policy)."""
+    SMOOTH = 0    # smallest 5-smooth size per axis
+    POW2 = 1      # smallest power of two per axis
+    AUTOTUNE = 2  # both timed on the GPU, fastest kept
+
+
+class MVDeconBatch:
+    """Iterate several independent deconvolutions (stacks) together
+    (mvd_batch_*): one launch per FFT pass and kernel class for all stacks,
+    optionally replayed as a hipGraph.  Each stack's result equals running it
+    alone.  The MVDeconvolution objects must be created with run=False and
+    stay alive while the batch is used."""
+
+    def __init__(self, deconvolutions: Sequence[MVDeconvolution], use_graph: bool = True, device: int = 0):
+        L = _native.lib()
+        h = ctypes.c_void_p()
+        _native.check(L.mvd_batch_create(device, ctypes.byref(h)), "mvd_batch_create")
+        self.handle = h
+        self.stacks = list(deconvolutions)
+        for d in self.stacks:
+            _native.check(L.mvd_batch_add(h, d._s.handle), "mvd_batch_add")
+        _native.check(L.mvd_batch_prepare(h, 1 if use_graph else 0), "mvd_batch_prepare")
+
+    def iterate(self, n: int):
+        """n batched iterations; returns per stack an array [n, views, 2] of statistics."""
+        if n <= 0:
+            return [np.zeros((0, d._s.nviews, 2)) for d in self.stacks]
+        tot = sum(d._s.nviews for d in self.stacks)
+        st = np.zeros((n, tot, 2), np.float64)
+        _native.check(_native.lib().mvd_batch_iterate(self.handle, n, _ptr(st)), "mvd_batch_iterate")
+        out, off = [], 0
+        for d in self.stacks:
+            nv = d._s.nviews
+            out.append(st[:, off:off + nv, :].copy())
+            d.stats.extend(list(out[-1]))
+            d.i += n
+            off += nv
+        return out
+
+    def close(self):
+        if self.handle:
+            _native.lib().mvd_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
