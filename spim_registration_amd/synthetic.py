@@ -177,3 +177,27 @@ def workload(name: str, with_ground_truth: bool = False, threads: Optional[int] 
     ds = make_dataset(w["size_xyz"], w["nviews"], w["angles"], w["nbeads"], w["k_xyz"], w["sxy"], w["sz"],
                       w["seed"], field=field, with_ground_truth=with_ground_truth, threads=threads)
     return ds, w
+
+
+def c3_stack_specs(seed: int = 3, nstacks: int = 32):
+    """Seeded stack parameters of the batched workload: per stack a size
+    uniform in {96, 128, ..., 384} per axis, a view count uniform in {2..8},
+    PSF sigma_z uniform in [3, 6] (sigma_xy 1.2, box 19x19x31) and beads at
+    200 per million voxels."""
+    u = uniform(seed, 0xC3, nstacks * 5)
+    sizes = list(range(96, 385, 32))
+    specs = []
+    for i in range(nstacks):
+        r = u[5 * i:5 * i + 5]
+        size = tuple(sizes[min(len(sizes) - 1, int(r[j] * len(sizes)))] for j in range(3))
+        nv = 2 + min(6, int(r[3] * 7))
+        sz = 3.0 + 3.0 * r[4]
+        nbeads = max(1, int(round(200 * size[0] * size[1] * size[2] / 1e6)))
+        specs.append(dict(size_xyz=size, nviews=nv, sz=sz, sxy=1.2, k_xyz=(19, 19, 31), nbeads=nbeads,
+                          angles=[360.0 * k / nv for k in range(nv)], seed=seed * 1000 + i))
+    return specs
+
+
+def c3_stack(spec, threads: Optional[int] = None) -> Dataset:
+    return make_dataset(spec["size_xyz"], spec["nviews"], spec["angles"], spec["nbeads"], spec["k_xyz"], spec["sxy"],
+                        spec["sz"], spec["seed"], threads=threads)
