@@ -180,6 +180,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain device copy of 256 MiB (counter calibration for traffic)")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the blocks of each view through the batch engine (one launch per pass, hipGraph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
@@ -208,12 +212,23 @@ def main():
                         weight_type=WeightType.PRECOMPUTED_WEIGHTS, run=False)
     log(f"prepared in {time.perf_counter() - t:.1f} s; geometry {d.geometry(0)}")
 
-    d.iterate(args.warmup)
+    if args.copy_probe:
+        src = torch.empty(64 * 2**20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = src.clone()  # one copy kernel: reads and writes 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
+    runner = d
+    if args.batched:
+        from spim_registration_amd import MVDeconBatch
+        runner = MVDeconBatch([d], use_graph=True)
+    runner.iterate(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    d.iterate(args.steps)
+    runner.iterate(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if args.batched:
+        runner.close()
     sec = t1 - t0
     n = int(np.prod(w["size_xyz"]))
     nv = w["nviews"]
