@@ -19,17 +19,9 @@
 #include <cstring>
 
 #include "mvd_internal.h"
+#include "pointwise_math.h"
 
 namespace mvd {
-
-__device__ __forceinline__ int mirror_idx(int p, int n) {
-  if (p >= 0 && p < n) return p;
-  if (n == 1) return 0;
-  const int period = 2 * n - 2;
-  p %= period;
-  if (p < 0) p += period;
-  return p < n ? p : period - p;
-}
 
 struct XFwdParams {
   ConvGeom g;
@@ -59,7 +51,7 @@ __device__ __forceinline__ float gather_value(const XFwdParams& P, int x, int y,
   if (inside) return P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
   if (P.gather == kGatherOne) return 1.f;
   if (P.gather == kGatherZero) return 0.f;
-  const int mx = mirror_idx(gx, g.src.x), my = mirror_idx(gy, g.src.y), mz = mirror_idx(gz, g.src.z);
+  const int mx = mirror_single(gx, g.src.x), my = mirror_single(gy, g.src.y), mz = mirror_single(gz, g.src.z);
   return P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
 }
 
@@ -211,30 +203,6 @@ __global__ __launch_bounds__(kThreads) void k_z_batch(const ZParams* items, cons
   z_body(items[item], bx, by);
 }
 
-// MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726),
-// evaluated with the reference's float/double cast points and no contraction.
-__device__ __forceinline__ float next_value(float last, float integral, float weight, double lambda) {
-#pragma clang fp contract(off)
-  const float minv = 0.0001f;
-  const float value = last * integral;
-  float adjusted;
-  if (value > 0.f) {
-    if (lambda > 0.0)
-      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
-    else
-      adjusted = value;
-  } else {
-    adjusted = minv;
-  }
-  const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
-  return last + ((next - last) * weight);
-}
-
-__device__ __forceinline__ float quotient(float blurred, float img) {
-#pragma clang fp contract(off)
-  return img > 0.f ? img / blurred : 1.f;
-}
-
 struct XInvParams {
   ConvGeom g;
   const float2* spec;
@@ -309,7 +277,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
         e.dst[gi] = 0.f;
       }
     } else if (e.mode == kEpiQuotient) {
-      e.dst[gi] = quotient(v, e.img[gi]);
+      e.dst[gi] = rl_quotient(v, e.img[gi]);
     } else {
       float w = 1.f;
       if (e.weight_mode == kWeightArray) {
@@ -319,7 +287,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
         w = (float)fmin(1.0, r * e.osem);
       }
       const float last = e.dst[gi];
-      const float next = next_value(last, v, w, e.lambda);
+      const float next = rl_next_value(last, v, w, e.lambda);
       e.dst[gi] = next;
       const float change = fabsf(next - last);
       my_sum += (double)change;
@@ -457,6 +425,31 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
   (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_x_inverse, dim3(n), dim3(kThreads), lds, stream, P);
   return hipGetLastError();
+}
+
+int conv_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g) {
+  return xy_fused(sp) ? g.eff_size.z : x_inverse_workgroups(sp, g);
+}
+
+hipError_t launch_xy_forward_passes(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather,
+                                    float2* spec, hipStream_t stream) {
+  if (xy_fused(sp)) return launch_xy_forward(sp, g, src, gather, spec, stream);
+  hipError_t e = launch_x_forward(sp, g, src, gather, spec, stream);
+  if (e != hipSuccess) return e;
+  return launch_y(sp, spec, 0, sp.q.z, 0, sp.q.y, -1.f, stream);
+}
+
+hipError_t launch_yx_inverse_passes(const ShapePlan& sp, const ConvGeom& g, float2* spec, const EpilogueArgs& e,
+                                    hipStream_t stream, int* nwg) {
+  if (xy_fused(sp)) {
+    if (nwg) *nwg = conv_inverse_workgroups(sp, g);
+    return launch_xy_inverse(sp, g, spec, e, stream);
+  }
+  const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
+  const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
+  hipError_t err = launch_y(sp, spec, zlo, zhi, ylo, yhi, 1.f, stream);
+  if (err != hipSuccess) return err;
+  return launch_x_inverse(sp, g, spec, e, stream, nwg);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,169 +1,14 @@
-// Register-resident FFT passes for power-of-two lengths (64, 128, 256, 512).
-//
-// Every line is transformed by T = N / E cooperating threads that each keep
-// E complex values in VGPRs.  A length is split into two or three radices
-// (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8); butterflies run
-// in registers and the lines meet in LDS only between radix passes -- one LDS
-// exchange for a 256-point line instead of one per radix-2/4 stage.  All
-// sizes are compile-time, so index arithmetic is shifts and masks.
-//
-// Stockham decimation in time: in a pass of radix R with Ns = product of the
-// earlier radices, butterfly j reads x[j + t N/R], multiplies by
-// exp(dir 2 pi i t (j mod Ns) / (Ns R)) and writes y[(j div Ns) Ns R + (j mod Ns)
-// + t Ns].  A thread owns butterflies j = jt + T b.  After the last pass the
-// thread holds X[jt + T b + t N/R_last], which is exactly the input order of a
-// transform with the radices reversed -- the fused z pass goes forward,
-// multiplies by the kernel spectrum and back without touching LDS in between.
+// Register-resident FFT passes for power-of-two lengths (64, 128, 256, 512):
+// the y pass, the fused z pass (forward, kernel product, inverse) and the
+// real-to-complex / complex-to-real x passes with their gather and epilogue.
+// The butterfly and LDS exchange building blocks live in fft_regs.h.
 #include <cstring>
 
+#include "fft_regs.h"
 #include "mvd_internal.h"
+#include "pointwise_math.h"
 
 namespace mvd {
-
-template <>
-__device__ __forceinline__ void butterfly<16>(float2* v, float dir) {
-  // n = 4 n2 + n1, k = k1 + 4 k2
-  const float c1 = 0.923879532511286756128183f;  // cos(pi/8)
-  const float s1 = 0.382683432365089771728460f;  // sin(pi/8)
-  const float h = 0.707106781186547524400844f;
-  float2 a[4][4];
-#pragma unroll
-  for (int n1 = 0; n1 < 4; ++n1) {
-    float2 t[4] = {v[n1], v[4 + n1], v[8 + n1], v[12 + n1]};
-    butterfly<4>(t, dir);
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) a[n1][k1] = t[k1];
-  }
-  // W16^m = cos(2 pi m / 16) + i dir sin(2 pi m / 16) for m = n1 k1
-  const float cs[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
-  const float sn[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
-#pragma unroll
-  for (int n1 = 1; n1 < 4; ++n1)
-#pragma unroll
-    for (int k1 = 1; k1 < 4; ++k1) {
-      const int m = n1 * k1;
-      a[n1][k1] = cmul(a[n1][k1], make_float2(cs[m], dir * sn[m]));
-    }
-#pragma unroll
-  for (int k1 = 0; k1 < 4; ++k1) {
-    float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
-    butterfly<4>(t, dir);
-#pragma unroll
-    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
-  }
-}
-
-template <int N>
-struct Spec;
-template <>
-struct Spec<64> {
-  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1;
-};
-template <>
-struct Spec<128> {
-  static constexpr int E = 16, R0 = 16, R1 = 8, R2 = 1;
-};
-template <>
-struct Spec<256> {
-  static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1;
-};
-template <>
-struct Spec<512> {
-  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8;
-};
-
-// Butterflies of one pass on the registers (inputs already in pass order).
-template <int N, int E, int R, int NS>
-__device__ __forceinline__ void bfly_pass(float2 (&v)[E], int jt, const float2* tw, int tw_step, float dir) {
-  constexpr int T = N / E;
-  constexpr int TS = N / (NS * R);
-#pragma unroll
-  for (int b = 0; b < E / R; ++b) {
-    if constexpr (NS > 1) {
-      const int k = (jt + T * b) & (NS - 1);
-#pragma unroll
-      for (int t = 1; t < R; ++t) {
-        float2 w = tw[t * k * TS * tw_step];
-        w.y *= -dir;
-        v[b * R + t] = cmul(v[b * R + t], w);
-      }
-    }
-    butterfly<R>(v + b * R, dir);
-  }
-}
-
-template <int N, int E, int R, int NS, class Addr>
-__device__ __forceinline__ void store_pass(const float2 (&v)[E], float2* lds, int jt, Addr addr) {
-  constexpr int T = N / E;
-#pragma unroll
-  for (int b = 0; b < E / R; ++b) {
-    const int j = jt + T * b;
-    const int base = (j / NS) * NS * R + (j & (NS - 1));
-#pragma unroll
-    for (int t = 0; t < R; ++t) lds[addr(base + t * NS)] = v[b * R + t];
-  }
-}
-
-template <int N, int E, int R, class Addr>
-__device__ __forceinline__ void load_pass(float2 (&v)[E], const float2* lds, int jt, Addr addr) {
-  constexpr int T = N / E;
-#pragma unroll
-  for (int b = 0; b < E / R; ++b)
-#pragma unroll
-    for (int t = 0; t < R; ++t) v[b * R + t] = lds[addr(jt + T * b + t * (N / R))];
-}
-
-// Element index held in v[b * R + t] when the pass of radix R reads or the
-// last pass of radix R writes.
-template <int N, int E, int R>
-__device__ __forceinline__ int elem_index(int jt, int b, int t) {
-  return jt + (N / E) * b + t * (N / R);
-}
-
-// Full transform.  v: inputs in the order of the first radix A; on return the
-// outputs in the order of the last radix.  Starts and ends with a barrier, so
-// LDS may be in use by the caller before and after.
-template <int N, int E, int A, int B, int C, class Addr>
-__device__ __forceinline__ void fft_regs(float2 (&v)[E], float2* lds, int jt, const float2* tw, int tw_step,
-                                         float dir, Addr addr) {
-  bfly_pass<N, E, A, 1>(v, jt, tw, tw_step, dir);
-  __syncthreads();
-  store_pass<N, E, A, 1>(v, lds, jt, addr);
-  __syncthreads();
-  load_pass<N, E, B>(v, lds, jt, addr);
-  bfly_pass<N, E, B, A>(v, jt, tw, tw_step, dir);
-  if constexpr (C > 1) {
-    __syncthreads();
-    store_pass<N, E, B, A>(v, lds, jt, addr);
-    __syncthreads();
-    load_pass<N, E, C>(v, lds, jt, addr);
-    bfly_pass<N, E, C, A * B>(v, jt, tw, tw_step, dir);
-  }
-  __syncthreads();
-}
-
-template <int N, class Addr>
-__device__ __forceinline__ void fft_forward_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
-                                                  int tw_step, float dir, Addr addr) {
-  using S = Spec<N>;
-  fft_regs<N, S::E, S::R0, S::R1, S::R2>(v, lds, jt, tw, tw_step, dir, addr);
-}
-
-// Radices reversed: input order = output order of fft_forward_order.
-template <int N, class Addr>
-__device__ __forceinline__ void fft_reverse_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
-                                                  int tw_step, float dir, Addr addr) {
-  using S = Spec<N>;
-  if constexpr (S::R2 > 1)
-    fft_regs<N, S::E, S::R2, S::R1, S::R0>(v, lds, jt, tw, tw_step, dir, addr);
-  else
-    fft_regs<N, S::E, S::R1, S::R0, 1>(v, lds, jt, tw, tw_step, dir, addr);
-}
-
-template <int N>
-constexpr int last_radix() {
-  return Spec<N>::R2 > 1 ? Spec<N>::R2 : Spec<N>::R1;
-}
 
 // ---------------------------------------------------------------------------
 // y pass: complex FFT along y of L consecutive kx columns for one z.
@@ -305,15 +150,6 @@ __global__ __launch_bounds__(kThreads) void k_z_fast_batch(const ZFastParams* it
 // row-major in LDS (coalesced global traffic); the FFT reads them with the
 // rows fastest across lanes.
 
-__device__ __forceinline__ int mirror_fast(int p, int n) {
-  if (p >= 0 && p < n) return p;
-  if (n == 1) return 0;
-  const int period = 2 * n - 2;
-  p %= period;
-  if (p < 0) p += period;
-  return p < n ? p : period - p;
-}
-
 struct XFwdFastParams {
   ConvGeom g;
   const float* src;
@@ -362,8 +198,8 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
         int gy = g.off.y + y, gz = g.off.z + z;
         row_inside = gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
         if (MODE == kGatherMirror && !row_inside) {
-          gy = mirror_fast(gy, g.src.y);
-          gz = mirror_fast(gz, g.src.z);
+          gy = mirror_single(gy, g.src.y);
+          gz = mirror_single(gz, g.src.z);
           row_inside = true;  // the mirrored row is a source row
         }
         srow = P.src + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz);
@@ -382,7 +218,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
           int gx = g.off.x + x;
           const bool in_x = gx >= 0 && gx < g.src.x;
           if (MODE == kGatherMirror) {
-            if (!in_x) gx = mirror_fast(gx, g.src.x);
+            if (!in_x) gx = mirror_single(gx, g.src.x);
             val = srow[gx];
           } else if (row_inside && in_x) {
             val = srow[gx];
@@ -454,28 +290,6 @@ struct XInvFastParams {
   const float2* tw;  // 2M entries
   int pitch;
 };
-
-__device__ __forceinline__ float next_value_f(float last, float integral, float weight, double lambda) {
-#pragma clang fp contract(off)
-  const float minv = 0.0001f;
-  const float value = last * integral;
-  float adjusted;
-  if (value > 0.f) {
-    if (lambda > 0.0)
-      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
-    else
-      adjusted = value;
-  } else {
-    adjusted = minv;
-  }
-  const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
-  return last + ((next - last) * weight);
-}
-
-__device__ __forceinline__ float quotient_f(float blurred, float img) {
-#pragma clang fp contract(off)
-  return img > 0.f ? img / blurred : 1.f;
-}
 
 template <int M, int EPI>
 __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int bx, int by) {
@@ -603,7 +417,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       if (EPI == kEpiStore) {
         e.dst[gi] = val;
       } else if (EPI == kEpiQuotient) {
-        e.dst[gi] = quotient_f(val, in0[r][ee]);
+        e.dst[gi] = rl_quotient(val, in0[r][ee]);
       } else if (EPI == kEpiResidual) {
         const double w = (double)in1[r][ee], b = (double)val, y = (double)in0[r][ee];
         if (b > 0) {
@@ -619,7 +433,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
           w = (float)fmin(1.0, rr * e.osem);
         }
         const float last = in0[r][ee];
-        const float next = next_value_f(last, val, w, e.lambda);
+        const float next = rl_next_value(last, val, w, e.lambda);
         e.dst[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;

@@ -1,0 +1,169 @@
+// Register-resident FFT building blocks shared by the fast per-axis passes
+// and the fused x-y plane passes.
+//
+// Every line is transformed by T = N / E cooperating threads that each keep
+// E complex values in VGPRs.  A length is split into two or three radices
+// (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8); butterflies run
+// in registers and the lines meet in LDS only between radix passes -- one LDS
+// exchange for a 256-point line instead of one per radix-2/4 stage.  All
+// sizes are compile-time, so index arithmetic is shifts and masks.
+//
+// Stockham decimation in time: in a pass of radix R with Ns = product of the
+// earlier radices, butterfly j reads x[j + t N/R], multiplies by
+// exp(dir 2 pi i t (j mod Ns) / (Ns R)) and writes y[(j div Ns) Ns R + (j mod Ns)
+// + t Ns].  A thread owns butterflies j = jt + T b.  After the last pass the
+// thread holds X[jt + T b + t N/R_last], which is exactly the input order of a
+// transform with the radices reversed -- the fused z pass goes forward,
+// multiplies by the kernel spectrum and back without touching LDS in between.
+#pragma once
+
+#include "mvd_internal.h"
+
+namespace mvd {
+
+template <>
+__device__ __forceinline__ void butterfly<16>(float2* v, float dir) {
+  // n = 4 n2 + n1, k = k1 + 4 k2
+  const float c1 = 0.923879532511286756128183f;  // cos(pi/8)
+  const float s1 = 0.382683432365089771728460f;  // sin(pi/8)
+  const float h = 0.707106781186547524400844f;
+  float2 a[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    float2 t[4] = {v[n1], v[4 + n1], v[8 + n1], v[12 + n1]};
+    butterfly<4>(t, dir);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) a[n1][k1] = t[k1];
+  }
+  // W16^m = cos(2 pi m / 16) + i dir sin(2 pi m / 16) for m = n1 k1
+  const float cs[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
+  const float sn[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
+#pragma unroll
+  for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) {
+      const int m = n1 * k1;
+      a[n1][k1] = cmul(a[n1][k1], make_float2(cs[m], dir * sn[m]));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    butterfly<4>(t, dir);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int N>
+struct Spec;
+template <>
+struct Spec<64> {
+  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1;
+};
+template <>
+struct Spec<128> {
+  static constexpr int E = 16, R0 = 16, R1 = 8, R2 = 1;
+};
+template <>
+struct Spec<256> {
+  static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1;
+};
+template <>
+struct Spec<512> {
+  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8;
+};
+
+// Butterflies of one pass on the registers (inputs already in pass order).
+template <int N, int E, int R, int NS>
+__device__ __forceinline__ void bfly_pass(float2 (&v)[E], int jt, const float2* tw, int tw_step, float dir) {
+  constexpr int T = N / E;
+  constexpr int TS = N / (NS * R);
+#pragma unroll
+  for (int b = 0; b < E / R; ++b) {
+    if constexpr (NS > 1) {
+      const int k = (jt + T * b) & (NS - 1);
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        float2 w = tw[t * k * TS * tw_step];
+        w.y *= -dir;
+        v[b * R + t] = cmul(v[b * R + t], w);
+      }
+    }
+    butterfly<R>(v + b * R, dir);
+  }
+}
+
+template <int N, int E, int R, int NS, class Addr>
+__device__ __forceinline__ void store_pass(const float2 (&v)[E], float2* lds, int jt, Addr addr) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int b = 0; b < E / R; ++b) {
+    const int j = jt + T * b;
+    const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+    for (int t = 0; t < R; ++t) lds[addr(base + t * NS)] = v[b * R + t];
+  }
+}
+
+template <int N, int E, int R, class Addr>
+__device__ __forceinline__ void load_pass(float2 (&v)[E], const float2* lds, int jt, Addr addr) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int b = 0; b < E / R; ++b)
+#pragma unroll
+    for (int t = 0; t < R; ++t) v[b * R + t] = lds[addr(jt + T * b + t * (N / R))];
+}
+
+// Element index held in v[b * R + t] when the pass of radix R reads or the
+// last pass of radix R writes.
+template <int N, int E, int R>
+__device__ __forceinline__ int elem_index(int jt, int b, int t) {
+  return jt + (N / E) * b + t * (N / R);
+}
+
+// Full transform.  v: inputs in the order of the first radix A; on return the
+// outputs in the order of the last radix.  Starts and ends with a barrier, so
+// LDS may be in use by the caller before and after.
+template <int N, int E, int A, int B, int C, class Addr>
+__device__ __forceinline__ void fft_regs(float2 (&v)[E], float2* lds, int jt, const float2* tw, int tw_step,
+                                         float dir, Addr addr) {
+  bfly_pass<N, E, A, 1>(v, jt, tw, tw_step, dir);
+  __syncthreads();
+  store_pass<N, E, A, 1>(v, lds, jt, addr);
+  __syncthreads();
+  load_pass<N, E, B>(v, lds, jt, addr);
+  bfly_pass<N, E, B, A>(v, jt, tw, tw_step, dir);
+  if constexpr (C > 1) {
+    __syncthreads();
+    store_pass<N, E, B, A>(v, lds, jt, addr);
+    __syncthreads();
+    load_pass<N, E, C>(v, lds, jt, addr);
+    bfly_pass<N, E, C, A * B>(v, jt, tw, tw_step, dir);
+  }
+  __syncthreads();
+}
+
+template <int N, class Addr>
+__device__ __forceinline__ void fft_forward_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
+                                                  int tw_step, float dir, Addr addr) {
+  using S = Spec<N>;
+  fft_regs<N, S::E, S::R0, S::R1, S::R2>(v, lds, jt, tw, tw_step, dir, addr);
+}
+
+// Radices reversed: input order = output order of fft_forward_order.
+template <int N, class Addr>
+__device__ __forceinline__ void fft_reverse_order(float2 (&v)[Spec<N>::E], float2* lds, int jt, const float2* tw,
+                                                  int tw_step, float dir, Addr addr) {
+  using S = Spec<N>;
+  if constexpr (S::R2 > 1)
+    fft_regs<N, S::E, S::R2, S::R1, S::R0>(v, lds, jt, tw, tw_step, dir, addr);
+  else
+    fft_regs<N, S::E, S::R1, S::R0, 1>(v, lds, jt, tw, tw_step, dir, addr);
+}
+
+template <int N>
+constexpr int last_radix() {
+  return Spec<N>::R2 > 1 ? Spec<N>::R2 : Spec<N>::R1;
+}
+
+}  // namespace mvd

@@ -1,0 +1,484 @@
+// Fused x + y passes for FFT boxes whose planes are 256 x 256 (q.x = q.y = 256).
+//
+// A 1024-thread workgroup owns one z plane.  Forward: the plane is gathered in
+// two halves of 128 rows; each half is transformed along x in LDS (real to
+// complex: a 128-point complex FFT of the packed even/odd samples, 8 threads
+// per row, then the unpack step) and every thread keeps the values of its two
+// y columns in registers (16 rows each).  The 129 columns then go through
+// 256-point FFTs along y, 64 columns per round, 16 threads per column.  The DC
+// and Nyquist columns are real after the x transform, so they share one
+// complex column (DC + i Nyquist) and are separated after the y transform; two
+// rounds cover all 129 columns.  Inverse: the same steps backwards -- y
+// transforms of the spectrum columns (DC and Nyquist packed again), rows
+// written back to LDS half by half, complex to real along x, fused epilogue.
+//
+// The plane never leaves the chip between the x and the y transform, which
+// saves one read and one write of the spectrum per direction compared with
+// separate x and y passes.  LDS: 256 x 65 complex for the y rounds, overlaid
+// with 128 x 129 complex for an x half, plus a 256-entry DC/Nyquist buffer and
+// the twiddle table -- about 137 KB, one workgroup per CU.
+#include <cstdlib>
+#include <cstring>
+
+#include "fft_regs.h"
+#include "mvd_internal.h"
+#include "pointwise_math.h"
+
+namespace mvd {
+
+namespace {
+
+constexpr int kXYThreads = 1024;
+constexpr int kXYM = 128;        // complex samples of a packed row (256 reals)
+constexpr int kXYN = 256;        // rows of a plane, length of the y transforms
+constexpr int kXYMP = kXYM + 1;  // LDS row pitch of an x half (complex)
+constexpr int kXYLP = 65;        // LDS pitch of a y round: 64 columns + 1
+constexpr int kXYHalf = 128;     // rows per x half
+constexpr int kXYLds = kXYN * kXYLP;
+constexpr int kXYEpiRows = 4;    // epilogue rows whose loads are issued together
+static_assert(kXYHalf * kXYMP <= kXYLds, "x half must fit the y-round buffer");
+
+using SX = Spec<kXYM>;  // 16 x 8: 16 values per thread, 8 threads per row
+using SY = Spec<kXYN>;  // 16 x 16: 16 values per thread, 16 threads per column
+static_assert(SX::E == 16 && SY::E == 16 && SX::R2 == 1 && SY::R2 == 1, "fused x-y radix layout");
+static_assert(kXYThreads / (kXYM / SX::E) == kXYHalf, "one x row per 8 threads");
+static_assert(kXYThreads / (kXYN / SY::E) == 64, "64 y columns per round");
+
+// Full transform of 16 values per thread; outputs in the order of the last
+// radix.  Barriers before the LDS exchange and after it.
+template <int N, class Addr>
+__device__ __forceinline__ void xy_fft(float2 (&v)[16], float2* lds, int jt, const float2* tw, int tw_step,
+                                       float dir, Addr addr) {
+  using S = Spec<N>;
+  bfly_pass<N, 16, S::R0, 1>(v, jt, tw, tw_step, dir);
+  __syncthreads();
+  store_pass<N, 16, S::R0, 1>(v, lds, jt, addr);
+  __syncthreads();
+  load_pass<N, 16, S::R1>(v, lds, jt, addr);
+  bfly_pass<N, 16, S::R1, S::R0>(v, jt, tw, tw_step, dir);
+  __syncthreads();
+}
+
+// 128-point transform of the rows of an x half, in place in LDS.
+__device__ __forceinline__ void xy_rows_fft(float2* lds, const float2* tw, float dir) {
+  const int line = threadIdx.x & (kXYHalf - 1), jt = threadIdx.x >> 7;
+  auto addr = [line](int i) { return line * kXYMP + i; };
+  float2 v[16];
+  load_pass<kXYM, 16, SX::R0>(v, lds, jt, addr);
+  xy_fft<kXYM>(v, lds, jt, tw, 2, dir, addr);
+#pragma unroll
+  for (int b = 0; b < 16 / SX::R1; ++b)
+#pragma unroll
+    for (int t = 0; t < SX::R1; ++t) lds[addr(elem_index<kXYM, 16, SX::R1>(jt, b, t))] = v[b * SX::R1 + t];
+}
+
+}  // namespace
+
+struct XYFwdParams {
+  ConvGeom g;
+  const float* src;
+  float2* spec;
+  const float2* tw;  // 256 entries exp(-2 pi i m / 256)
+  int pitch;
+};
+
+template <int MODE>
+__device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
+  __shared__ float2 lds[kXYLds];
+  __shared__ float2 dcn[kXYN];
+  __shared__ float2 tw[kXYN];
+  const int tid = threadIdx.x;
+  const ConvGeom& g = P.g;
+  const int z = bx;
+  if (tid < kXYN) tw[tid] = P.tw[tid];
+  float* fbuf = reinterpret_cast<float*>(lds);
+  // y columns of this thread: cl and 64 + cl, rows jy + 16 t
+  const int cl = tid & 63, jy = tid >> 6;
+  float2 c0[16], c1[16];
+  const bool zlive = z < g.win.z;
+  int gz = g.off.z + z;
+  bool z_in = gz >= 0 && gz < g.src.z;
+  if (MODE == kGatherMirror && !z_in) {
+    gz = mirror_single(gz, g.src.z);
+    z_in = true;
+  }
+  const int x = tid & 255, rsub = tid >> 8;
+  int gx = g.off.x + x;
+  const bool x_in = gx >= 0 && gx < g.src.x;
+  if (MODE == kGatherMirror && !x_in) gx = mirror_single(gx, g.src.x);
+  const bool x_live = x < g.win.x;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    // gather 128 rows of 256 real samples; lanes run along x, all loads first
+    float vals[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const int y = h * kXYHalf + rsub + 4 * i;
+      float val = 0.f;
+      if (zlive && x_live && y < g.win.y) {
+        int gy = g.off.y + y;
+        const bool y_in = gy >= 0 && gy < g.src.y;
+        if (MODE == kGatherMirror) {
+          if (!y_in) gy = mirror_single(gy, g.src.y);
+          val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+        } else if (z_in && y_in && x_in) {
+          val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+        } else {
+          val = MODE == kGatherOne ? 1.f : 0.f;
+        }
+      }
+      vals[i] = val;
+    }
+    __syncthreads();  // the previous half's column reads are done
+#pragma unroll
+    for (int i = 0; i < 32; ++i) fbuf[(rsub + 4 * i) * 2 * kXYMP + x] = vals[i];
+    __syncthreads();
+    xy_rows_fft(lds, tw, -1.f);
+    __syncthreads();
+    // unpack Z = FFT(even + i odd) into X[k] and X[M - k], in place
+    for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
+      const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
+      float2* row = lds + l * kXYMP;
+      const float2 a = row[k], b = row[(kXYM - k) & (kXYM - 1)];
+      const float2 bc = cconj(b), ac = cconj(a);
+      const float2 E1 = cscale(cadd(a, bc), 0.5f), D1 = cscale(csub(a, bc), 0.5f);
+      const float2 E2 = cscale(cadd(b, ac), 0.5f), D2 = cscale(csub(b, ac), 0.5f);
+      const float2 xk = cadd(E1, cmul(tw[k], make_float2(D1.y, -D1.x)));
+      const float2 xm = cadd(E2, cmul(tw[kXYM - k], make_float2(D2.y, -D2.x)));
+      row[k] = xk;
+      if (kXYM - k != k) row[kXYM - k] = xm;
+    }
+    __syncthreads();
+    // this half's rows of my columns; column 0 carries DC + i Nyquist
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const float2* row = lds + (jy + 16 * t) * kXYMP;
+      c0[t + 8 * h] = cl == 0 ? make_float2(row[0].x, row[kXYM].x) : row[cl];
+      c1[t + 8 * h] = row[64 + cl];
+    }
+  }
+  // y transforms, 64 columns per round
+  float2* out = P.spec + (int64_t)z * kXYN * P.pitch;
+  auto yaddr = [cl](int i) { return i * kXYLP + cl; };
+  xy_fft<kXYN>(c0, lds, jy, tw, 1, -1.f, yaddr);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
+    if (cl == 0)
+      dcn[ky] = c0[t];
+    else
+      out[(int64_t)ky * P.pitch + cl] = c0[t];
+  }
+  xy_fft<kXYN>(c1, lds, jy, tw, 1, -1.f, yaddr);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
+    out[(int64_t)ky * P.pitch + 64 + cl] = c1[t];
+  }
+  // separate the DC and Nyquist columns: Z = F0 + i F128 with F0 and F128 the
+  // transforms of real columns (the dcn writes are ordered by the barriers of
+  // the second round)
+  if (tid < kXYN) {
+    const int k = tid;
+    const float2 a = dcn[k], b = cconj(dcn[(kXYN - k) & (kXYN - 1)]);
+    const float2 s = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
+    out[(int64_t)k * P.pitch] = s;
+    out[(int64_t)k * P.pitch + kXYM] = make_float2(d.y, -d.x);
+  }
+}
+
+struct XYInvParams {
+  ConvGeom g;
+  const float2* spec;
+  EpilogueArgs e;
+  const float2* tw;
+  int pitch;
+};
+
+template <int EPI>
+__device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
+  __shared__ float2 lds[kXYLds];
+  __shared__ float2 nyq[kXYN];
+  __shared__ float2 tw[kXYN];
+  __shared__ double red_sum[kXYThreads / 64];
+  __shared__ float red_max[kXYThreads / 64];
+  const int tid = threadIdx.x;
+  const ConvGeom& g = P.g;
+  const EpilogueArgs& e = P.e;
+  const int zo = bx;                 // output plane inside the effective box
+  const int z = g.eff_local.z + zo;  // plane of the FFT box
+  if (tid < kXYN) tw[tid] = P.tw[tid];
+  const int cl = tid & 63, jy = tid >> 6;
+  const float2* in = P.spec + (int64_t)z * kXYN * P.pitch;
+  auto yaddr = [cl](int i) { return i * kXYLP + cl; };
+  float2 c0[16], c1[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const float2* row = in + (int64_t)elem_index<kXYN, 16, SY::R0>(jy, 0, t) * P.pitch;
+    c0[t] = row[cl];
+    c1[t] = row[64 + cl];
+  }
+  if (tid < kXYN) nyq[tid] = in[(int64_t)tid * P.pitch + kXYM];
+  __syncthreads();
+  if (cl == 0) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {  // column 0 carries DC + i Nyquist
+      const float2 b = nyq[elem_index<kXYN, 16, SY::R0>(jy, 0, t)];
+      c0[t] = make_float2(c0[t].x - b.y, c0[t].y + b.x);
+    }
+  }
+  xy_fft<kXYN>(c0, lds, jy, tw, 1, 1.f, yaddr);
+  xy_fft<kXYN>(c1, lds, jy, tw, 1, 1.f, yaddr);
+  double my_sum = 0.0;
+  float my_max = -1.f;
+  const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
+  const int nx = g.eff_size.x;
+  const float* fbuf = reinterpret_cast<const float*>(lds);
+  const int x = tid & 255, rsub = tid >> 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // earlier readers of lds are done
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      float2* row = lds + (jy + 16 * t) * kXYMP;
+      const float2 a = c0[t + 8 * h];
+      if (cl == 0) {
+        row[0] = make_float2(a.x, 0.f);
+        row[kXYM] = make_float2(a.y, 0.f);
+      } else {
+        row[cl] = a;
+      }
+      row[64 + cl] = c1[t + 8 * h];
+    }
+    __syncthreads();
+    // pack X[k], X[M - k] into Z[k] = E + i O
+    for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
+      const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
+      float2* row = lds + l * kXYMP;
+      const float2 a = row[k], b = row[kXYM - k];
+      const float2 za = cadd(cadd(a, cconj(b)), mul_di(cmul(csub(a, cconj(b)), cconj(tw[k])), 1.f));
+      if (k != 0 && kXYM - k != k) {
+        const float2 zb = cadd(cadd(b, cconj(a)), mul_di(cmul(csub(b, cconj(a)), cconj(tw[kXYM - k])), 1.f));
+        row[kXYM - k] = zb;
+      }
+      row[k] = za;
+    }
+    __syncthreads();
+    xy_rows_fft(lds, tw, 1.f);
+    __syncthreads();
+    // epilogue over this half's output rows; lanes run along x
+    if (x < nx) {
+#pragma unroll
+      for (int c8 = 0; c8 < 32 / kXYEpiRows; ++c8) {
+        int64_t gidx[kXYEpiRows];
+        float in0[kXYEpiRows], in1[kXYEpiRows];
+#pragma unroll
+        for (int i = 0; i < kXYEpiRows; ++i) {
+          const int y = h * kXYHalf + rsub + 4 * (c8 * kXYEpiRows + i);
+          gidx[i] = -1;
+          in0[i] = 0.f;
+          in1[i] = 1.f;
+          if (y >= ylo && y < yhi) {
+            const int64_t gi = (int64_t)(g.eff_offset.x + x) +
+                               (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + y - ylo) +
+                                                   (int64_t)g.dst.y * (g.eff_offset.z + zo));
+            gidx[i] = gi;
+            if (EPI == kEpiQuotient || EPI == kEpiResidual) {
+              in0[i] = e.img[gi];
+              if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = e.weight[gi];
+            } else if (EPI == kEpiUpdate) {
+              in0[i] = e.dst[gi];
+              if (e.weight_mode != kWeightNone) in1[i] = e.weight[gi];
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < kXYEpiRows; ++i) {
+          const int64_t gi = gidx[i];
+          if (gi < 0) continue;
+          const int yl = rsub + 4 * (c8 * kXYEpiRows + i);
+          const float val = fbuf[yl * 2 * kXYMP + g.eff_local.x + x];
+          if (EPI == kEpiStore) {
+            e.dst[gi] = val;
+          } else if (EPI == kEpiQuotient) {
+            e.dst[gi] = rl_quotient(val, in0[i]);
+          } else if (EPI == kEpiResidual) {
+            const double w = (double)in1[i], b = (double)val, yv = (double)in0[i];
+            if (b > 0) {
+              my_sum += w * (b - yv * log(b));
+              e.dst[gi] = (float)(w * (1.0 - yv / b));
+            } else {
+              e.dst[gi] = 0.f;
+            }
+          } else {
+            float w = in1[i];
+            if (e.weight_mode == kWeightVirtual) {
+              const double r = (double)in1[i] / (double)e.wsum[gi];
+              w = (float)fmin(1.0, r * e.osem);
+            }
+            const float last = in0[i];
+            const float next = rl_next_value(last, val, w, e.lambda);
+            e.dst[gi] = next;
+            const float change = fabsf(next - last);
+            my_sum += (double)change;
+            my_max = fmaxf(my_max, change);
+          }
+        }
+      }
+    }
+  }
+  if (EPI == kEpiUpdate || EPI == kEpiResidual) {
+    for (int o = 32; o > 0; o >>= 1) {
+      my_sum += __shfl_xor(my_sum, o);
+      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+    }
+    const int wid = tid / 64;
+    if ((tid & 63) == 0) {
+      red_sum[wid] = my_sum;
+      red_max[wid] = my_max;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      float m = -1.f;
+      for (int i = 0; i < kXYThreads / 64; ++i) {
+        s += red_sum[i];
+        m = fmaxf(m, red_max[i]);
+      }
+      e.part_sum[bx] = s;
+      e.part_max[bx] = m;
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kXYThreads) void k_xy_forward(XYFwdParams P) {
+  xy_forward_body<MODE>(P, blockIdx.x);
+}
+template <int MODE>
+__global__ __launch_bounds__(kXYThreads) void k_xy_forward_batch(const XYFwdParams* items, const int* prefix,
+                                                                const int* gx, int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  xy_forward_body<MODE>(items[item], bx);
+}
+template <int EPI>
+__global__ __launch_bounds__(kXYThreads) void k_xy_inverse(XYInvParams P) {
+  xy_inverse_body<EPI>(P, blockIdx.x);
+}
+template <int EPI>
+__global__ __launch_bounds__(kXYThreads) void k_xy_inverse_batch(const XYInvParams* items, const int* prefix,
+                                                                const int* gx, int nitems) {
+  int item, bx, by;
+  batch_locate(prefix, gx, nitems, item, bx, by);
+  xy_inverse_body<EPI>(items[item], bx);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+// MVD_DISABLE_FUSED_XY=1 keeps the separate x and y passes (for comparisons).
+const bool g_xy_disabled = [] {
+  const char* v = std::getenv("MVD_DISABLE_FUSED_XY");
+  return v && v[0] == '1';
+}();
+
+bool xy_gather_ok(int gather) { return gather == kGatherMirror || gather == kGatherOne || gather == kGatherZero; }
+}  // namespace
+
+bool xy_fused(const ShapePlan& sp) { return !g_xy_disabled && sp.q.x == kXYN && sp.q.y == kXYN; }
+
+hipError_t launch_xy_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                             hipStream_t stream) {
+  if (!xy_fused(sp) || !xy_gather_ok(gather) || g.q.x != kXYN || g.q.y != kXYN) return hipErrorInvalidValue;
+  XYFwdParams P{g, src, spec, sp.tw_x, sp.pitch};
+  (void)hipGetLastError();
+  const dim3 grid(g.q.z), block(kXYThreads);
+  switch (gather) {
+    case kGatherMirror: hipLaunchKernelGGL(k_xy_forward<kGatherMirror>, grid, block, 0, stream, P); break;
+    case kGatherOne: hipLaunchKernelGGL(k_xy_forward<kGatherOne>, grid, block, 0, stream, P); break;
+    default: hipLaunchKernelGGL(k_xy_forward<kGatherZero>, grid, block, 0, stream, P); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_xy_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                             hipStream_t stream) {
+  if (!xy_fused(sp) || g.q.x != kXYN || g.q.y != kXYN) return hipErrorInvalidValue;
+  if (g.eff_size.z <= 0) return hipSuccess;
+  XYInvParams P{g, spec, e, sp.tw_x, sp.pitch};
+  (void)hipGetLastError();
+  const dim3 grid(g.eff_size.z), block(kXYThreads);
+  switch (e.mode) {
+    case kEpiStore: hipLaunchKernelGGL(k_xy_inverse<kEpiStore>, grid, block, 0, stream, P); break;
+    case kEpiQuotient: hipLaunchKernelGGL(k_xy_inverse<kEpiQuotient>, grid, block, 0, stream, P); break;
+    case kEpiUpdate: hipLaunchKernelGGL(k_xy_inverse<kEpiUpdate>, grid, block, 0, stream, P); break;
+    case kEpiResidual: hipLaunchKernelGGL(k_xy_inverse<kEpiResidual>, grid, block, 0, stream, P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int build_xy_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) {
+  const ShapePlan& sp = *d.sp;
+  *gy = 1;
+  if (d.kind == kPassXForward) {
+    static_assert(sizeof(XYFwdParams) <= kMaxParamBytes, "parameter block");
+    XYFwdParams P{d.g, d.src, d.spec, sp.tw_x, sp.pitch};
+    *gx = d.g.q.z;
+    *cls = d.gather;
+    std::memcpy(out, &P, sizeof(P));
+    return sizeof(P);
+  }
+  static_assert(sizeof(XYInvParams) <= kMaxParamBytes, "parameter block");
+  XYInvParams P{d.g, d.spec, d.e, sp.tw_x, sp.pitch};
+  *gx = d.g.eff_size.z;
+  *cls = d.e.mode;
+  std::memcpy(out, &P, sizeof(P));
+  return sizeof(P);
+}
+
+hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
+                           int nwg, hipStream_t stream) {
+  if (nwg <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  const dim3 grid(nwg), block(kXYThreads);
+  if (kind == kPassXForward) {
+    const XYFwdParams* it = static_cast<const XYFwdParams*>(items);
+    switch (cls) {
+      case kGatherMirror:
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherMirror>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      case kGatherOne:
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherOne>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      case kGatherZero:
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherZero>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      default: return hipErrorInvalidValue;
+    }
+  } else if (kind == kPassXInverse) {
+    const XYInvParams* it = static_cast<const XYInvParams*>(items);
+    switch (cls) {
+      case kEpiStore:
+        hipLaunchKernelGGL(k_xy_inverse_batch<kEpiStore>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      case kEpiQuotient:
+        hipLaunchKernelGGL(k_xy_inverse_batch<kEpiQuotient>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      case kEpiUpdate:
+        hipLaunchKernelGGL(k_xy_inverse_batch<kEpiUpdate>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      case kEpiResidual:
+        hipLaunchKernelGGL(k_xy_inverse_batch<kEpiResidual>, grid, block, 0, stream, it, prefix, gx, nitems);
+        break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mvd

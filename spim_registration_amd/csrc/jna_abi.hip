@@ -221,11 +221,9 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
   EpilogueArgs e{};
   e.mode = kEpiStore;
   e.dst = c->block;
-  JNA_HIP(launch_x_forward(*sp, g, c->block, kGatherZero, c->spec, c->stream));
-  JNA_HIP(launch_y(*sp, c->spec, 0, q.z, 0, q.y, -1.f, c->stream));
+  JNA_HIP(launch_xy_forward_passes(*sp, g, c->block, kGatherZero, c->spec, c->stream));
   JNA_HIP(launch_z(*sp, c->spec, ks->spec, 1.f, 0, bd.z, c->stream));
-  JNA_HIP(launch_y(*sp, c->spec, 0, bd.z, 0, bd.y, 1.f, c->stream));
-  JNA_HIP(launch_x_inverse(*sp, g, c->spec, e, c->stream, nullptr));
+  JNA_HIP(launch_yx_inverse_passes(*sp, g, c->spec, e, c->stream, nullptr));
   JNA_HIP(hipMemcpyAsync(out, c->block, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream));
   JNA_HIP(hipStreamSynchronize(c->stream));
   return true;

@@ -115,6 +115,17 @@ hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2
                             hipStream_t stream, int* nwg);
 int x_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g);
 
+// The x and y passes of a convolution window, fused per plane when q.x = q.y =
+// 256 (fft_xy.hip) and separate otherwise.  Forward: x then y over the whole
+// box (gather rules mirror / one / zero).  Inverse: y over rows and planes of
+// the effective box, then x with the epilogue; *nwg = number of statistics
+// partials written (conv_inverse_workgroups).
+hipError_t launch_xy_forward_passes(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather,
+                                    float2* spec, hipStream_t stream);
+hipError_t launch_yx_inverse_passes(const ShapePlan& sp, const ConvGeom& g, float2* spec, const EpilogueArgs& e,
+                                    hipStream_t stream, int* nwg);
+int conv_inverse_workgroups(const ShapePlan& sp, const ConvGeom& g);
+
 hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, int n, double* out,
                                hipStream_t stream);
 hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
@@ -138,6 +149,13 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                                  hipStream_t stream, int nwg);
 
+// Fused x-y plane kernels (fft_xy.hip).
+bool xy_fused(const ShapePlan& sp);
+hipError_t launch_xy_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
+                             hipStream_t stream);
+hipError_t launch_xy_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
+                             hipStream_t stream);
+
 // Batched passes (batch.hip): one launch runs the same pass for many items
 // (blocks of many stacks), each item with its own geometry and buffers.
 enum PassKind : int { kPassXForward = 0, kPassYForward, kPassZConv, kPassYInverse, kPassXInverse };
@@ -152,6 +170,7 @@ struct PassDesc {
   EpilogueArgs e;       // x inverse epilogue
   int zlo, zhi, ylo, yhi;
   float dir, scale;
+  int xy;  // x passes of a fused x-y item (no separate y pass)
 };
 constexpr int kMaxParamBytes = 640;
 bool pass_is_fast(const PassDesc& d);
@@ -164,6 +183,10 @@ hipError_t launch_batch_generic(int kind, int epi, const void* items, const int*
                                 int nwg, size_t lds, hipStream_t stream);
 hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
                              int nwg, hipStream_t stream);
+// Fused x-y items: kind kPassXForward (cls = gather) or kPassXInverse (cls = epilogue).
+int build_xy_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls);
+hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
+                           int nwg, hipStream_t stream);
 struct ReduceItem {
   const double* part_sum;
   const float* part_max;

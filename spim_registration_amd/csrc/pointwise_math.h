@@ -1,0 +1,46 @@
+// Per-voxel arithmetic shared by every gather and epilogue: the mirror-single
+// out-of-bounds rule, the Richardson-Lucy quotient and the Tikhonov-regularised
+// update, evaluated with the reference's float/double cast points and with FMA
+// contraction off so that every kernel rounds the same way.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mvd {
+
+// Mirror-single extension of the block copy (Views.extendMirrorSingle,
+// deconv/MVDeconFFTThreads.java:44,78): period 2n - 2, edges not repeated.
+__device__ __forceinline__ int mirror_single(int p, int n) {
+  if (p >= 0 && p < n) return p;
+  if (n == 1) return 0;
+  const int period = 2 * n - 2;
+  p %= period;
+  if (p < 0) p += period;
+  return p < n ? p : period - p;
+}
+
+// MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726).
+__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda) {
+#pragma clang fp contract(off)
+  const float minv = 0.0001f;
+  const float value = last * integral;
+  float adjusted;
+  if (value > 0.f) {
+    if (lambda > 0.0)
+      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
+    else
+      adjusted = value;
+  } else {
+    adjusted = minv;
+  }
+  const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
+  return last + ((next - last) * weight);
+}
+
+// Quotient of the observed view by the blurred estimate (MVDeconvolution.computeQuotient, deconv/MVDeconvolution.java:494-546).
+__device__ __forceinline__ float rl_quotient(float blurred, float img) {
+#pragma clang fp contract(off)
+  return img > 0.f ? img / blurred : 1.f;
+}
+
+}  // namespace mvd

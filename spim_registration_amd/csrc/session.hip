@@ -185,12 +185,17 @@ enum TimingClass {
   kTZForward,
   kTReduce,
   kTPointwise,
+  kTXYForward,
+  kTXYInvQuotient,
+  kTXYInvUpdate,
+  kTXYInvStore,
   kTNumClasses
 };
 static const char* kTimingNames[kTNumClasses] = {"x_forward",         "y_forward",       "z_conv",
                                                  "y_inverse",         "x_inverse_quotient", "x_inverse_update",
                                                  "x_inverse_store",   "z_forward",       "reduce_stats",
-                                                 "pointwise"};
+                                                 "pointwise",         "xy_forward",      "xy_inverse_quotient",
+                                                 "xy_inverse_update", "xy_inverse_store"};
 
 class Timer {
  public:
@@ -438,6 +443,18 @@ int run_conv(mvd_session* s, const ShapePlan& sp, const ConvGeom& g, const float
   Timer& T = s->timer;
   const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
   const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
+  if (xy_fused(sp)) {
+    T.begin(st);
+    MVD_CHECK(launch_xy_forward(sp, g, src, gather, spec, st));
+    T.end(st, kTXYForward);
+    T.begin(st);
+    MVD_CHECK(launch_z(sp, spec, khat, 1.f, zlo, zhi, st));
+    T.end(st, kTZConv);
+    T.begin(st);
+    MVD_CHECK(launch_xy_inverse(sp, g, spec, e, st));
+    T.end(st, e.mode == kEpiQuotient ? kTXYInvQuotient : e.mode == kEpiUpdate ? kTXYInvUpdate : kTXYInvStore);
+    return 0;
+  }
   T.begin(st);
   MVD_CHECK(launch_x_forward(sp, g, src, gather, spec, st));
   T.end(st, kTXForward);
@@ -677,7 +694,7 @@ int build_geometry(mvd_session* s, int v) {
   MVD_REQUIRE(divide_into_blocks(s->img, view.kd, bs, q, &view.blocks),
               "block size smaller than the kernel (BlockGeneratorFixedSizePrecise)");
   view.nwg_update = 0;
-  for (auto& g : view.blocks) view.nwg_update += x_inverse_workgroups(*view.plan, g);
+  for (auto& g : view.blocks) view.nwg_update += conv_inverse_workgroups(*view.plan, g);
   return 0;
 }
 
@@ -980,7 +997,7 @@ int mvd_iterate(mvd_session* s, int n, double* stats) {
         u.part_max = s->d_part_max.as<float>() + off;
         int rc = run_conv(s, sp, g, ratio, kGatherOne, view.k2hat.as<float2>(), u, spec);
         if (rc) return rc;
-        off += x_inverse_workgroups(sp, g);
+        off += conv_inverse_workgroups(sp, g);
       }
       s->timer.begin(s->stream);
       MVD_CHECK(launch_reduce_stats(s->d_part_sum.as<double>(), s->d_part_max.as<float>(), off,
@@ -1215,7 +1232,8 @@ struct mvd_batch {
   std::vector<std::vector<DeviceBuf>> spec;  // [stack][block]
   struct Launch {
     int kind = 0, cls = 0, epi = 0, nitems = 0, nwg = 0;
-    bool fast = false, reduce = false;
+    int path = 0;  // 0 generic kernels, 1 register-resident kernels, 2 fused x-y kernels
+    bool reduce = false;
     size_t lds = 0;
     DeviceBuf items, prefix, gx;
   };
@@ -1236,23 +1254,27 @@ struct mvd_batch {
 namespace {
 
 int add_pass_launches(mvd_batch* b, const std::vector<PassDesc>& descs) {
-  // group by kernel class
-  std::map<std::pair<int, int>, std::vector<const PassDesc*>> groups;  // (fast ? cls : -1, epi) -> items
+  // group by kernel class: (path, template selector, epilogue) -> items
+  std::map<std::tuple<int, int, int>, std::vector<const PassDesc*>> groups;
   for (auto& d : descs) {
-    int cls = -1;
-    if (pass_is_fast(d)) {
-      unsigned char tmp[kMaxParamBytes];
-      int gx, gy;
+    int cls = -1, path = 0;
+    unsigned char tmp[kMaxParamBytes];
+    int gx, gy;
+    if (d.xy) {
+      path = 2;
+      build_xy_params(d, tmp, &gx, &gy, &cls);
+    } else if (pass_is_fast(d)) {
+      path = 1;
       build_fast_params(d, tmp, &gx, &gy, &cls);
     }
-    groups[{cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
+    groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
   }
   for (auto& kv : groups) {
     mvd_batch::Launch L;
     L.kind = kv.second.front()->kind;
-    L.fast = kv.first.first >= 0;
-    L.cls = kv.first.first;
-    L.epi = kv.first.second;
+    L.path = std::get<0>(kv.first);
+    L.cls = std::get<1>(kv.first);
+    L.epi = std::get<2>(kv.first);
     std::vector<unsigned char> blob;
     std::vector<int> prefix(1, 0), gxs;
     int psize = 0;
@@ -1260,7 +1282,9 @@ int add_pass_launches(mvd_batch* b, const std::vector<PassDesc>& descs) {
       unsigned char tmp[kMaxParamBytes];
       int gx = 0, gy = 0, cls = 0;
       size_t lds = 0;
-      const int n = L.fast ? build_fast_params(*d, tmp, &gx, &gy, &cls) : build_generic_params(*d, tmp, &gx, &gy, &lds);
+      const int n = L.path == 2   ? build_xy_params(*d, tmp, &gx, &gy, &cls)
+                    : L.path == 1 ? build_fast_params(*d, tmp, &gx, &gy, &cls)
+                                  : build_generic_params(*d, tmp, &gx, &gy, &lds);
       MVD_REQUIRE(n > 0 && n <= kMaxParamBytes, "internal: parameter block size");
       psize = n;
       blob.insert(blob.end(), tmp, tmp + n);
@@ -1286,7 +1310,10 @@ int enqueue_batch_iteration(mvd_batch* b) {
   for (auto& L : b->launches) {
     if (L.reduce) {
       MVD_CHECK(launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, b->stream));
-    } else if (L.fast) {
+    } else if (L.path == 2) {
+      MVD_CHECK(launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
+                                b->stream));
+    } else if (L.path == 1) {
       MVD_CHECK(launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
                                   b->stream));
     } else {
@@ -1384,8 +1411,10 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
           d.gather = conv == 0 ? kGatherMirror : kGatherOne;
           d.kspec = conv == 0 ? view.k1hat.as<float2>() : view.k2hat.as<float2>();
           d.scale = 1.f;
+          const int fused = xy_fused(*view.plan) ? 1 : 0;
           PassDesc x = d;
           x.kind = kPassXForward;
+          x.xy = fused;
           pass[0].push_back(x);
           PassDesc yf = d;
           yf.kind = kPassYForward;
@@ -1394,7 +1423,7 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
           yf.ylo = 0;
           yf.yhi = view.plan->q.y;
           yf.dir = -1.f;
-          pass[1].push_back(yf);
+          if (!fused) pass[1].push_back(yf);
           PassDesc z = d;
           z.kind = kPassZConv;
           z.zlo = g.eff_local.z;
@@ -1408,9 +1437,10 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
           yi.ylo = g.eff_local.y;
           yi.yhi = g.eff_local.y + g.eff_size.y;
           yi.dir = 1.f;
-          pass[3].push_back(yi);
+          if (!fused) pass[3].push_back(yi);
           PassDesc xi = d;
           xi.kind = kPassXInverse;
+          xi.xy = fused;
           if (conv == 0) {
             xi.e.mode = kEpiQuotient;
             xi.e.dst = s->d_ratio.as<float>();
@@ -1425,7 +1455,7 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
             xi.e.lambda = s->lambda;
             xi.e.part_sum = s->d_part_sum.as<double>() + off;
             xi.e.part_max = s->d_part_max.as<float>() + off;
-            off += x_inverse_workgroups(*view.plan, g);
+            off += conv_inverse_workgroups(*view.plan, g);
           }
           pass[4].push_back(xi);
         }
@@ -1441,7 +1471,7 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
       mvd_session* s = b->stacks[si];
       if (v >= s->nviews) continue;
       int n = 0;
-      for (auto& g : s->views[v].blocks) n += x_inverse_workgroups(*s->views[v].plan, g);
+      for (auto& g : s->views[v].blocks) n += conv_inverse_workgroups(*s->views[v].plan, g);
       red.push_back({s->d_part_sum.as<double>(), s->d_part_max.as<float>(), n,
                      b->stats_iter.as<double>() + b->stats_off[si] + 2 * v});
     }

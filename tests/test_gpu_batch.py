@@ -25,7 +25,7 @@ def _stack(size, nv, seed, k=(7, 7, 9), block=None, policy=0, run=False, iters=0
 
 
 SPECS = [((40, 36, 24), 3, None), ((57, 30, 44), 2, None), ((64, 64, 48), 4, (64, 64, 64)),
-         ((100, 90, 70), 2, (64, 64, 64))]
+         ((100, 90, 70), 2, (64, 64, 64)), ((260, 250, 20), 2, (256, 256, 32))]
 
 
 @pytest.mark.parametrize("use_graph", [False, True])

@@ -172,6 +172,22 @@ def test_weighted_deconvolution_matches_oracle(wtype, osem_index):
     d.close()
 
 
+@pytest.mark.parametrize("lam,block", [(0.006, (256, 256, 32)), (0.0, (256, 256, 24))])
+def test_fused_plane_blocks_match_oracle(lam, block):
+    """FFT boxes with 256 x 256 planes take the fused x-y plane kernels; blocks
+    on all sides of the image exercise the mirror and constant-one gathers."""
+    ds = S.make_dataset((300, 270, 20), 2, [0, 90], 40, (7, 7, 9), 1.2, 2.5, seed=13)
+    print("fused deconv", lam, block, flush=True)
+    d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 3, lam, block=block)
+    geo = d.geometry(0)
+    assert tuple(geo["fft"][:2]) == (256, 256) and geo["blocks"] >= 4
+    psi = d.get_psi()
+    ref, st = O.deconvolve(ds.views, ds.psfs, O.EFFICIENT_BAYESIAN, 3, lam, block_xyz=block, norm_threads=8)
+    assert rel_max(psi, ref) < GATE
+    np.testing.assert_allclose(np.array(d.stats)[:, :, 0], st[:, :, 0], rtol=1e-3)
+    d.close()
+
+
 def test_initial_psi_and_resume():
     ds = _small_dataset(seed=4)
     d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 2, 0.0)
