@@ -47,7 +47,7 @@ def kernel_bytes(geom_blocks, q, img_n):
     H = hx * qy * qz
     P = qx * qy * qz
     b = dict(x_forward=0.0, y_forward=0.0, z_conv=0.0, y_inverse=0.0, x_inverse_quotient=0.0,
-             x_inverse_update=0.0)
+             x_inverse_update=0.0, xy_forward=0.0, xy_inverse_quotient=0.0, xy_inverse_update=0.0)
     for (ex, ey, ez) in geom_blocks:
         E = ex * ey * ez
         b["x_forward"] += 2 * (4 * P + 8 * H)
@@ -57,6 +57,11 @@ def kernel_bytes(geom_blocks, q, img_n):
         rows = hx * ey * ez * 8
         b["x_inverse_quotient"] += rows + 8 * E
         b["x_inverse_update"] += rows + 12 * E
+        # fused x-y plane kernels (256 x 256 planes): gather + spectrum write;
+        # spectrum planes of the output box + epilogue
+        b["xy_forward"] += 2 * (4 * P + 8 * H)
+        b["xy_inverse_quotient"] += 8 * H * ez / qz + 8 * E
+        b["xy_inverse_update"] += 8 * H * ez / qz + 12 * E
     return b
 
 
