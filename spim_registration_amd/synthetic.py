@@ -179,6 +179,15 @@ def workload(name: str, with_ground_truth: bool = False, threads: Optional[int] 
     return ds, w
 
 
+def perturbed_psfs(k_xyz, sxy: float, sz: float, angles: Sequence[float], seed: int,
+                   rel: float = 0.2) -> List[np.ndarray]:
+    """Start PSFs of the refinement workload: every view's sigma_xy and sigma_z
+    scaled by an independent seeded factor uniform in [1 - rel, 1 + rel]."""
+    u = uniform(seed, 0xC4, 2 * len(angles))
+    return [psf(k_xyz, sxy * (1 + rel * (2 * u[2 * i] - 1)), sz * (1 + rel * (2 * u[2 * i + 1] - 1)), a)
+            for i, a in enumerate(angles)]
+
+
 def c3_stack_specs(seed: int = 3, nstacks: int = 32):
     """Seeded stack parameters of the batched workload: per stack a size
     uniform in {96, 128, ..., 384} per axis, a view count uniform in {2..8},
