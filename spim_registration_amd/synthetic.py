@@ -111,18 +111,22 @@ def _blend_lut() -> np.ndarray:
 def blending_weight(shape_zyx, border_xyz=(-8, -8, -3), blending_xyz=(12, 12, 12)) -> np.ndarray:
     """Blending weight of a view on its own (identity-transformed) grid."""
     lut = _blend_lut()
-    w = np.ones(shape_zyx, np.float32)
+    f = []
     for d, ax in ((0, 2), (1, 1), (2, 0)):
         n = shape_zyx[ax]
         l = np.arange(n, dtype=np.float32)
         dist = np.maximum(np.float32(0), np.minimum(l - np.float32(border_xyz[d]),
                                                      np.float32(n - 1) - l - np.float32(border_xyz[d])))
         rel = dist / np.float32(blending_xyz[d])
-        f = np.where(rel < 1, lut[np.floor(rel.astype(np.float64) * 1000.0 + 0.5).clip(0, 1000).astype(int)], 1.0)
-        f = np.where(dist == 0, 0.0, f).astype(np.float64)
-        shp = [1, 1, 1]
-        shp[ax] = n
-        w = (w.astype(np.float64) * f.reshape(shp)).astype(np.float32)
+        fa = np.where(rel < 1, lut[np.floor(rel.astype(np.float64) * 1000.0 + 0.5).clip(0, 1000).astype(int)], 1.0)
+        f.append(np.where(dist == 0, 0.0, fa).astype(np.float64))
+    fx, fy, fz = f
+    # float products in the order x, y, z, each rounded to float32
+    plane = (fx.astype(np.float32).astype(np.float64)[None, :] * fy[:, None]).astype(np.float32)
+    w = np.empty(shape_zyx, np.float32)
+    p64 = plane.astype(np.float64)
+    for z in range(shape_zyx[0]):
+        w[z] = (p64 * fz[z]).astype(np.float32)
     return w
 
 
