@@ -203,20 +203,21 @@ def run_c4(args):
     d.iterate(2)
     nv = w["nviews"]
     n = int(np.prod(w["size_xyz"]))
-    # gradient throughput (inputs resident; NLL partials and gradients returned to the host)
+    # gradient throughput at the kernels the iteration uses (inputs resident;
+    # NLL partials and gradients returned to the host)
+    kernels = [v.get_kernel1() for v in d.get_data().get_views()]
     for _ in range(args.warmup):
-        d.nll_grad()
+        d.nll_grad(psfs=kernels)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        nll, gpsi, gpsf = d.nll_grad()
+        nll, gpsi, gpsf = d.nll_grad(psfs=kernels)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     value = n * nv * args.steps / sec / 1e9
     log(f"{args.steps} gradient evaluations in {sec:.3f} s -> {value:.3f} Gvoxel·view·grad-eval/s")
     # FD check of the PSF gradient at this point
     psi = d.get_psi(masked=False)
-    kernels = [v.get_kernel1() for v in d.get_data().get_views()]
     rng = np.random.default_rng(w["seed"])
     fd_err = []
     t = time.perf_counter()
@@ -243,6 +244,14 @@ def run_c4(args):
     log(f"{args.refine_steps} refinement steps in {rsec:.2f} s; PSF rel. error {e0:.4f} -> {e1:.4f}; "
         f"NLL {hist[0].nll:.6e} -> {hist[-1].nll:.6e}")
     d.close()
+    # path model per view and gradient evaluation: forward convolution, PSF
+    # correlation and adjoint convolution over the window img + K - 1, plus the
+    # residual and gradient epilogues: 3 (4P + 24H) + 16E
+    win = [a + b - 1 for a, b in zip(w["size_xyz"], w["k_xyz"])]
+    P = int(np.prod(win))
+    H = (win[0] // 2 + 1) * win[1] * win[2]
+    model = 3 * (4 * P + 24 * H) + 16 * n
+    path_gbs = model * nv * args.steps / sec / 1e9
     out = {
         "metric": "c4 likelihood-gradient throughput (NLL + d/dpsi + d/dPSF, 4 views)",
         "value": round(value, 4),
@@ -258,6 +267,9 @@ def run_c4(args):
         "data": "synthetic",
         "config": {"workload": "c4: 4 views of 256x256x256 f32, PSF 31x31x31, start sigma x U[0.8, 1.2], "
                                "whole image as one block", "views": nv},
+        "path_roofline": {"bound": "hbm", "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(path_gbs / HBM_PEAK_GBS, 4),
+                          "model": "3 (4P + 24H) + 16E bytes per view and gradient evaluation, P = (img + K - 1)^3"},
         "fd_check": {"voxels_per_view": 64, "h": 1e-3, "reference": "FP64 CPU oracle central differences",
                      "max_rel_err_per_view": [round(e, 6) for e in fd_err], "gate": 1e-3,
                      "pass": bool(max(fd_err) <= 1e-3)},

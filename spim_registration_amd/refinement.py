@@ -61,13 +61,16 @@ def refine_psfs(decon: MVDeconvolution, steps: int, iterations: int = 1, eta: fl
     for step in range(steps):
         decon.iterate(iterations)
         psi = decon.get_psi(masked=False)
-        nll, _, gpsf = decon.nll_grad(want_psi=False, want_psf=True)
+        # the gradient at the kernels the iteration uses (kernel1: the given PSF
+        # after normImg), not at the PSFs as handed in
+        kernels = [view.get_kernel1() for view in views]
+        nll, _, gpsf = decon.nll_grad(psfs=kernels, want_psi=False, want_psf=True)
         change = 0.0
         new_psfs = []
         for v, view in enumerate(views):
             w = None if weights is None else weights[v]
             total = float(psi.astype(np.float64).sum() if w is None else (psi.astype(np.float64) * w).sum())
-            old = view.get_kernel1()
+            old = kernels[v]
             new = em_psf_update(old, gpsf[v], total, eta)
             change = max(change, float(np.abs(new - old).max() / max(np.abs(old).max(), 1e-30)))
             new_psfs.append(new)
