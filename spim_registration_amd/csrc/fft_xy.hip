@@ -35,7 +35,7 @@ constexpr int kXYMP = kXYM + 1;  // LDS row pitch of an x half (complex)
 constexpr int kXYLP = 65;        // LDS pitch of a y round: 64 columns + 1
 constexpr int kXYHalf = 128;     // rows per x half
 constexpr int kXYLds = kXYN * kXYLP;
-constexpr int kXYEpiRows = 4;    // epilogue rows whose loads are issued together
+constexpr int kXYEpiRows = 16;   // epilogue rows whose loads are issued together
 static_assert(kXYHalf * kXYMP <= kXYLds, "x half must fit the y-round buffer");
 
 using SX = Spec<kXYM>;  // 16 x 8: 16 values per thread, 8 threads per row
@@ -107,10 +107,8 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
   const bool x_in = gx >= 0 && gx < g.src.x;
   if (MODE == kGatherMirror && !x_in) gx = mirror_single(gx, g.src.x);
   const bool x_live = x < g.win.x;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    // gather 128 rows of 256 real samples; lanes run along x, all loads first
-    float vals[32];
+  // gather of one half: 128 rows of 256 real samples, lanes along x
+  auto gather = [&](int h, float (&vals)[32]) {
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
       const int y = h * kXYHalf + rsub + 4 * i;
@@ -129,9 +127,16 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
       }
       vals[i] = val;
     }
+  };
+  float vals[32];
+  gather(0, vals);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
     __syncthreads();  // the previous half's column reads are done
 #pragma unroll
     for (int i = 0; i < 32; ++i) fbuf[(rsub + 4 * i) * 2 * kXYMP + x] = vals[i];
+    // the second half's loads stay in flight while the first half is transformed
+    if (h == 0) gather(1, vals);
     __syncthreads();
     xy_rows_fft(lds, tw, -1.f);
     __syncthreads();
@@ -235,6 +240,67 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int nx = g.eff_size.x;
   const float* fbuf = reinterpret_cast<const float*>(lds);
   const int x = tid & 255, rsub = tid >> 8;
+  const bool x_live = x < nx;
+  constexpr int R = kXYEpiRows;
+  auto out_index = [&](int y, int zo_) -> int64_t {
+    return (int64_t)(g.eff_offset.x + x) +
+           (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + y - ylo) + (int64_t)g.dst.y * (g.eff_offset.z + zo_));
+  };
+  // inputs of the epilogue for rows rsub + 4 (c R + i) of half h
+  auto load_batch = [&](int h, int c, float (&in0)[R], float (&in1)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int y = h * kXYHalf + rsub + 4 * (c * R + i);
+      in0[i] = 0.f;
+      in1[i] = 1.f;
+      if (x_live && y >= ylo && y < yhi) {
+        const int64_t gi = out_index(y, zo);
+        if (EPI == kEpiQuotient || EPI == kEpiResidual) {
+          in0[i] = e.img[gi];
+          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = e.weight[gi];
+        } else if (EPI == kEpiUpdate) {
+          in0[i] = e.dst[gi];
+          if (e.weight_mode != kWeightNone) in1[i] = e.weight[gi];
+        }
+      }
+    }
+  };
+  auto run_batch = [&](int h, int c, const float (&in0)[R], const float (&in1)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int yl = rsub + 4 * (c * R + i);
+      const int y = h * kXYHalf + yl;
+      if (!x_live || y < ylo || y >= yhi) continue;
+      const int64_t gi = out_index(y, zo);
+      const float val = fbuf[yl * 2 * kXYMP + g.eff_local.x + x];
+      if (EPI == kEpiStore) {
+        e.dst[gi] = val;
+      } else if (EPI == kEpiQuotient) {
+        e.dst[gi] = rl_quotient(val, in0[i]);
+      } else if (EPI == kEpiResidual) {
+        const double w = (double)in1[i], b = (double)val, yv = (double)in0[i];
+        if (b > 0) {
+          my_sum += w * (b - yv * log(b));
+          e.dst[gi] = (float)(w * (1.0 - yv / b));
+        } else {
+          e.dst[gi] = 0.f;
+        }
+      } else {
+        float w = in1[i];
+        if (e.weight_mode == kWeightVirtual) {
+          const double r = (double)in1[i] / (double)e.wsum[gi];
+          w = (float)fmin(1.0, r * e.osem);
+        }
+        const float last = in0[i];
+        const float next = rl_next_value(last, val, w, e.lambda);
+        e.dst[gi] = next;
+        const float change = fabsf(next - last);
+        my_sum += (double)change;
+        my_max = fmaxf(my_max, change);
+      }
+    }
+  };
+  float p0[R], p1[R];  // first batch of a half, loaded ahead of the transform where registers allow
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     __syncthreads();  // earlier readers of lds are done
@@ -250,6 +316,8 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       }
       row[64 + cl] = c1[t + 8 * h];
     }
+    // the second half holds no y columns in registers any more
+    if (h == 1) load_batch(1, 0, p0, p1);
     __syncthreads();
     // pack X[k], X[M - k] into Z[k] = E + i O
     for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
@@ -267,64 +335,13 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     xy_rows_fft(lds, tw, 1.f);
     __syncthreads();
     // epilogue over this half's output rows; lanes run along x
-    if (x < nx) {
+    if (h == 0) load_batch(0, 0, p0, p1);
+    run_batch(h, 0, p0, p1);
 #pragma unroll
-      for (int c8 = 0; c8 < 32 / kXYEpiRows; ++c8) {
-        int64_t gidx[kXYEpiRows];
-        float in0[kXYEpiRows], in1[kXYEpiRows];
-#pragma unroll
-        for (int i = 0; i < kXYEpiRows; ++i) {
-          const int y = h * kXYHalf + rsub + 4 * (c8 * kXYEpiRows + i);
-          gidx[i] = -1;
-          in0[i] = 0.f;
-          in1[i] = 1.f;
-          if (y >= ylo && y < yhi) {
-            const int64_t gi = (int64_t)(g.eff_offset.x + x) +
-                               (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + y - ylo) +
-                                                   (int64_t)g.dst.y * (g.eff_offset.z + zo));
-            gidx[i] = gi;
-            if (EPI == kEpiQuotient || EPI == kEpiResidual) {
-              in0[i] = e.img[gi];
-              if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = e.weight[gi];
-            } else if (EPI == kEpiUpdate) {
-              in0[i] = e.dst[gi];
-              if (e.weight_mode != kWeightNone) in1[i] = e.weight[gi];
-            }
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < kXYEpiRows; ++i) {
-          const int64_t gi = gidx[i];
-          if (gi < 0) continue;
-          const int yl = rsub + 4 * (c8 * kXYEpiRows + i);
-          const float val = fbuf[yl * 2 * kXYMP + g.eff_local.x + x];
-          if (EPI == kEpiStore) {
-            e.dst[gi] = val;
-          } else if (EPI == kEpiQuotient) {
-            e.dst[gi] = rl_quotient(val, in0[i]);
-          } else if (EPI == kEpiResidual) {
-            const double w = (double)in1[i], b = (double)val, yv = (double)in0[i];
-            if (b > 0) {
-              my_sum += w * (b - yv * log(b));
-              e.dst[gi] = (float)(w * (1.0 - yv / b));
-            } else {
-              e.dst[gi] = 0.f;
-            }
-          } else {
-            float w = in1[i];
-            if (e.weight_mode == kWeightVirtual) {
-              const double r = (double)in1[i] / (double)e.wsum[gi];
-              w = (float)fmin(1.0, r * e.osem);
-            }
-            const float last = in0[i];
-            const float next = rl_next_value(last, val, w, e.lambda);
-            e.dst[gi] = next;
-            const float change = fabsf(next - last);
-            my_sum += (double)change;
-            my_max = fmaxf(my_max, change);
-          }
-        }
-      }
+    for (int c = 1; c < 32 / R; ++c) {
+      float a0[R], a1[R];
+      load_batch(h, c, a0, a1);
+      run_batch(h, c, a0, a1);
     }
   }
   if (EPI == kEpiUpdate || EPI == kEpiResidual) {

@@ -379,6 +379,21 @@ static std::vector<float> inverted(const std::vector<float>& k, Int3 d) {
 
 }  // namespace mvd
 
+namespace mvd {
+
+// One launch of a pass over a group of items (blocks of one or many stacks)
+// that share the kernel class; items, prefix and per-item grids live on the device.
+struct PassLaunch {
+  int kind = 0, cls = 0, epi = 0, nitems = 0, nwg = 0;
+  int path = 0;    // 0 generic kernels, 1 register-resident kernels, 2 fused x-y kernels
+  int timing = 0;  // TimingClass of the launch
+  bool reduce = false;
+  size_t lds = 0;
+  DeviceBuf items, prefix, gx;
+};
+
+}  // namespace mvd
+
 using namespace mvd;
 
 struct mvd_session {
@@ -406,6 +421,15 @@ struct mvd_session {
   DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
   PlanCache plans;
   Timer timer;
+  // The blocks of a view convolved together: one launch per pass and kernel
+  // class for all blocks (one spectrum buffer per block).  Built at the first
+  // iteration after prepare / new PSFs / new weights; off when the extra
+  // spectra do not fit (the per-block loop then reuses d_spec).
+  struct BlockBatch {
+    bool built = false, enabled = true;
+    std::vector<DeviceBuf> spec;
+    std::vector<std::vector<PassLaunch>> view;  // per view: convolve1 passes, then convolve2 passes
+  } bb;
 
   int fft_policy = MVD_FFT_SMOOTH;
   int psf_type = MVD_PSF_EFFICIENT_BAYESIAN;
@@ -423,6 +447,187 @@ struct mvd_session {
 };
 
 namespace {
+
+int timing_class(int kind, int path, int epi) {
+  switch (kind) {
+    case kPassXForward: return path == 2 ? kTXYForward : kTXForward;
+    case kPassYForward: return kTYForward;
+    case kPassZConv: return kTZConv;
+    case kPassYInverse: return kTYInverse;
+    default:
+      if (path == 2) return epi == kEpiQuotient ? kTXYInvQuotient : epi == kEpiUpdate ? kTXYInvUpdate : kTXYInvStore;
+      return epi == kEpiQuotient ? kTXInvQuotient : epi == kEpiUpdate ? kTXInvUpdate : kTXInvStore;
+  }
+}
+
+// Groups the items of one pass by kernel class -- (path, template selector,
+// epilogue) -- and appends one launch per group.
+int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaunch>* out) {
+  std::map<std::tuple<int, int, int>, std::vector<const PassDesc*>> groups;
+  for (auto& d : descs) {
+    int cls = -1, path = 0;
+    unsigned char tmp[kMaxParamBytes];
+    int gx, gy;
+    if (d.xy) {
+      path = 2;
+      build_xy_params(d, tmp, &gx, &gy, &cls);
+    } else if (pass_is_fast(d)) {
+      path = 1;
+      build_fast_params(d, tmp, &gx, &gy, &cls);
+    }
+    groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
+  }
+  for (auto& kv : groups) {
+    PassLaunch L;
+    L.kind = kv.second.front()->kind;
+    L.path = std::get<0>(kv.first);
+    L.cls = std::get<1>(kv.first);
+    L.epi = std::get<2>(kv.first);
+    L.timing = timing_class(L.kind, L.path, L.epi);
+    std::vector<unsigned char> blob;
+    std::vector<int> prefix(1, 0), gxs;
+    for (const PassDesc* d : kv.second) {
+      unsigned char tmp[kMaxParamBytes];
+      int gx = 0, gy = 0, cls = 0;
+      size_t lds = 0;
+      const int n = L.path == 2   ? build_xy_params(*d, tmp, &gx, &gy, &cls)
+                    : L.path == 1 ? build_fast_params(*d, tmp, &gx, &gy, &cls)
+                                  : build_generic_params(*d, tmp, &gx, &gy, &lds);
+      MVD_REQUIRE(n > 0 && n <= kMaxParamBytes, "internal: parameter block size");
+      blob.insert(blob.end(), tmp, tmp + n);
+      gxs.push_back(std::max(gx, 1));
+      prefix.push_back(prefix.back() + gx * gy);
+      L.lds = std::max(L.lds, lds);
+    }
+    L.nitems = (int)kv.second.size();
+    L.nwg = prefix.back();
+    MVD_CHECK(L.items.ensure(blob.size()));
+    MVD_CHECK(hipMemcpy(L.items.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    MVD_CHECK(L.prefix.ensure(sizeof(int) * prefix.size()));
+    MVD_CHECK(hipMemcpy(L.prefix.p, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice));
+    MVD_CHECK(L.gx.ensure(sizeof(int) * gxs.size()));
+    MVD_CHECK(hipMemcpy(L.gx.p, gxs.data(), sizeof(int) * gxs.size(), hipMemcpyHostToDevice));
+    out->push_back(std::move(L));
+  }
+  return 0;
+}
+
+hipError_t enqueue_pass_launch(const PassLaunch& L, hipStream_t st) {
+  if (L.reduce) return launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, st);
+  if (L.path == 2)
+    return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
+  if (L.path == 1)
+    return launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
+  return launch_batch_generic(L.kind, L.epi, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, L.lds,
+                              st);
+}
+
+// The pass items of one convolution of view v over all its blocks: convolve1
+// (conv 0: psi -> ratio, quotient epilogue) or convolve2 (conv 1: ratio -> psi,
+// update epilogue with statistics partials), block spectra in spec[block].
+void conv_pass_descs(mvd_session* s, int v, int conv, const std::vector<float2*>& spec,
+                     std::vector<PassDesc> (&pass)[5]) {
+  auto& view = s->views[v];
+  const int wm = s->weight_mode == MVD_WEIGHTS_NONE ? kWeightNone
+                 : s->weight_mode == MVD_WEIGHTS_VIRTUAL ? kWeightVirtual
+                                                         : kWeightArray;
+  int off = 0;
+  for (size_t bi = 0; bi < view.blocks.size(); ++bi) {
+    const ConvGeom& g = view.blocks[bi];
+    PassDesc d{};
+    d.sp = view.plan;
+    d.g = g;
+    d.spec = spec[bi];
+    d.src = conv == 0 ? s->d_psi.as<float>() : s->d_ratio.as<float>();
+    d.gather = conv == 0 ? kGatherMirror : kGatherOne;
+    d.kspec = conv == 0 ? view.k1hat.as<float2>() : view.k2hat.as<float2>();
+    d.scale = 1.f;
+    const int fused = xy_fused(*view.plan) ? 1 : 0;
+    PassDesc x = d;
+    x.kind = kPassXForward;
+    x.xy = fused;
+    pass[0].push_back(x);
+    PassDesc yf = d;
+    yf.kind = kPassYForward;
+    yf.zlo = 0;
+    yf.zhi = view.plan->q.z;
+    yf.ylo = 0;
+    yf.yhi = view.plan->q.y;
+    yf.dir = -1.f;
+    if (!fused) pass[1].push_back(yf);
+    PassDesc z = d;
+    z.kind = kPassZConv;
+    z.zlo = g.eff_local.z;
+    z.zhi = g.eff_local.z + g.eff_size.z;
+    z.dir = -1.f;
+    pass[2].push_back(z);
+    PassDesc yi = d;
+    yi.kind = kPassYInverse;
+    yi.zlo = g.eff_local.z;
+    yi.zhi = g.eff_local.z + g.eff_size.z;
+    yi.ylo = g.eff_local.y;
+    yi.yhi = g.eff_local.y + g.eff_size.y;
+    yi.dir = 1.f;
+    if (!fused) pass[3].push_back(yi);
+    PassDesc xi = d;
+    xi.kind = kPassXInverse;
+    xi.xy = fused;
+    if (conv == 0) {
+      xi.e.mode = kEpiQuotient;
+      xi.e.dst = s->d_ratio.as<float>();
+      xi.e.img = s->d_views.as<float>() + v * s->n;
+    } else {
+      xi.e.mode = kEpiUpdate;
+      xi.e.dst = s->d_psi.as<float>();
+      xi.e.weight_mode = wm;
+      xi.e.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
+      xi.e.wsum = s->d_wsum.as<float>();
+      xi.e.osem = s->osem;
+      xi.e.lambda = s->lambda;
+      xi.e.part_sum = s->d_part_sum.as<double>() + off;
+      xi.e.part_max = s->d_part_max.as<float>() + off;
+      off += conv_inverse_workgroups(*view.plan, g);
+    }
+    pass[4].push_back(xi);
+  }
+}
+
+// Launch lists of the block-batched iteration (one spectrum buffer per block).
+// Off for single-block views and when the extra spectra would take more than
+// half of the free device memory.
+int build_block_batch(mvd_session* s) {
+  auto& bb = s->bb;
+  bb.view.clear();
+  size_t nb = 0, elems = 0;
+  for (auto& v : s->views) {
+    nb = std::max(nb, v.blocks.size());
+    elems = std::max<size_t>(elems, v.plan->spectrum_elems());
+  }
+  size_t have = 0, free_b = 0, total_b = 0;
+  for (auto& b : bb.spec) have += b.bytes;
+  MVD_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const size_t need = nb * elems * sizeof(float2);
+  bb.built = true;
+  bb.enabled = nb > 1 && need <= have + free_b / 2;
+  if (!bb.enabled) return 0;
+  bb.spec.resize(nb);
+  std::vector<float2*> spec;
+  for (auto& b : bb.spec) {
+    MVD_CHECK(b.ensure(sizeof(float2) * elems));
+    spec.push_back(b.as<float2>());
+  }
+  bb.view.resize(s->nviews);
+  for (int v = 0; v < s->nviews; ++v)
+    for (int conv = 0; conv < 2; ++conv) {
+      std::vector<PassDesc> pass[5];
+      conv_pass_descs(s, v, conv, spec, pass);
+      for (int p = 0; p < 5; ++p) {
+        int rc = build_pass_launches(pass[p], &bb.view[v]);
+        if (rc) return rc;
+      }
+    }
+  return 0;
+}
 
 struct DeviceGuard {
   int prev = -1;
@@ -812,6 +1017,7 @@ int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value,
   MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
   MVD_REQUIRE(mode >= MVD_WEIGHTS_NONE && mode <= MVD_WEIGHTS_VIRTUAL, "invalid weight mode");
   MVD_REQUIRE(norm_threads >= 1, "norm_threads must be >= 1");
+  s->bb.built = false;
   bool all_w = true;
   for (auto& v : s->views) all_w = all_w && v.has_weight;
   MVD_REQUIRE(mode == MVD_WEIGHTS_NONE || all_w, "this weight mode needs a weight image for every view");
@@ -880,6 +1086,7 @@ int mvd_prepare(mvd_session* s, int psf_type, double lambda, const int* block_xy
   MVD_REQUIRE(psf_type >= 0 && psf_type <= 3, "invalid PSF type");
   MVD_REQUIRE(norm_threads >= 1, "norm_threads must be >= 1");
   DeviceGuard guard(s->device);
+  s->bb.built = false;
   s->psf_type = psf_type;
   s->lambda = lambda;
   s->norm_threads = norm_threads;
@@ -927,6 +1134,7 @@ int mvd_set_psf(mvd_session* s, int view, const float* psf) {
   auto& v = s->views[view];
   v.psf_raw.assign(psf, psf + v.k1.size());
   v.k1 = v.psf_raw;
+  s->bb.built = false;
   int rc = init_view(s, view);
   if (rc) return rc;
   return view_spectra(s, view);
@@ -969,7 +1177,26 @@ int mvd_iterate(mvd_session* s, int n, double* stats) {
   const int wm = s->weight_mode == MVD_WEIGHTS_NONE ? kWeightNone
                  : s->weight_mode == MVD_WEIGHTS_VIRTUAL ? kWeightVirtual
                                                          : kWeightArray;
-  for (int it = 0; it < n; ++it) {
+  if (!s->bb.built) {
+    int rc = build_block_batch(s);
+    if (rc) return rc;
+  }
+  // blocks of a view together: per pass one launch for all blocks
+  for (int it = 0; s->bb.enabled && it < n; ++it) {
+    for (int v = 0; v < s->nviews; ++v) {
+      for (auto& L : s->bb.view[v]) {
+        s->timer.begin(s->stream);
+        MVD_CHECK(enqueue_pass_launch(L, s->stream));
+        s->timer.end(s->stream, L.timing);
+      }
+      s->timer.begin(s->stream);
+      MVD_CHECK(launch_reduce_stats(s->d_part_sum.as<double>(), s->d_part_max.as<float>(), s->views[v].nwg_update,
+                                    s->d_stats.as<double>() + 2 * ((size_t)it * s->nviews + v), s->stream));
+      s->timer.end(s->stream, kTReduce);
+    }
+  }
+  // block after block through one spectrum buffer
+  for (int it = 0; !s->bb.enabled && it < n; ++it) {
     for (int v = 0; v < s->nviews; ++v) {
       auto& view = s->views[v];
       const ShapePlan& sp = *view.plan;
@@ -1230,14 +1457,7 @@ struct mvd_batch {
   hipStream_t stream = nullptr;
   std::vector<mvd_session*> stacks;
   std::vector<std::vector<DeviceBuf>> spec;  // [stack][block]
-  struct Launch {
-    int kind = 0, cls = 0, epi = 0, nitems = 0, nwg = 0;
-    int path = 0;  // 0 generic kernels, 1 register-resident kernels, 2 fused x-y kernels
-    bool reduce = false;
-    size_t lds = 0;
-    DeviceBuf items, prefix, gx;
-  };
-  std::vector<Launch> launches;
+  std::vector<PassLaunch> launches;
   DeviceBuf stats_iter, stats_hist;
   std::vector<int> stats_off;  // per stack, in doubles
   int total_stats = 0;
@@ -1253,74 +1473,8 @@ struct mvd_batch {
 
 namespace {
 
-int add_pass_launches(mvd_batch* b, const std::vector<PassDesc>& descs) {
-  // group by kernel class: (path, template selector, epilogue) -> items
-  std::map<std::tuple<int, int, int>, std::vector<const PassDesc*>> groups;
-  for (auto& d : descs) {
-    int cls = -1, path = 0;
-    unsigned char tmp[kMaxParamBytes];
-    int gx, gy;
-    if (d.xy) {
-      path = 2;
-      build_xy_params(d, tmp, &gx, &gy, &cls);
-    } else if (pass_is_fast(d)) {
-      path = 1;
-      build_fast_params(d, tmp, &gx, &gy, &cls);
-    }
-    groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
-  }
-  for (auto& kv : groups) {
-    mvd_batch::Launch L;
-    L.kind = kv.second.front()->kind;
-    L.path = std::get<0>(kv.first);
-    L.cls = std::get<1>(kv.first);
-    L.epi = std::get<2>(kv.first);
-    std::vector<unsigned char> blob;
-    std::vector<int> prefix(1, 0), gxs;
-    int psize = 0;
-    for (const PassDesc* d : kv.second) {
-      unsigned char tmp[kMaxParamBytes];
-      int gx = 0, gy = 0, cls = 0;
-      size_t lds = 0;
-      const int n = L.path == 2   ? build_xy_params(*d, tmp, &gx, &gy, &cls)
-                    : L.path == 1 ? build_fast_params(*d, tmp, &gx, &gy, &cls)
-                                  : build_generic_params(*d, tmp, &gx, &gy, &lds);
-      MVD_REQUIRE(n > 0 && n <= kMaxParamBytes, "internal: parameter block size");
-      psize = n;
-      blob.insert(blob.end(), tmp, tmp + n);
-      gxs.push_back(std::max(gx, 1));
-      prefix.push_back(prefix.back() + gx * gy);
-      L.lds = std::max(L.lds, lds);
-    }
-    (void)psize;
-    L.nitems = (int)kv.second.size();
-    L.nwg = prefix.back();
-    MVD_CHECK(L.items.ensure(blob.size()));
-    MVD_CHECK(hipMemcpy(L.items.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    MVD_CHECK(L.prefix.ensure(sizeof(int) * prefix.size()));
-    MVD_CHECK(hipMemcpy(L.prefix.p, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice));
-    MVD_CHECK(L.gx.ensure(sizeof(int) * gxs.size()));
-    MVD_CHECK(hipMemcpy(L.gx.p, gxs.data(), sizeof(int) * gxs.size(), hipMemcpyHostToDevice));
-    b->launches.push_back(std::move(L));
-  }
-  return 0;
-}
-
 int enqueue_batch_iteration(mvd_batch* b) {
-  for (auto& L : b->launches) {
-    if (L.reduce) {
-      MVD_CHECK(launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, b->stream));
-    } else if (L.path == 2) {
-      MVD_CHECK(launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
-                                b->stream));
-    } else if (L.path == 1) {
-      MVD_CHECK(launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
-                                  b->stream));
-    } else {
-      MVD_CHECK(launch_batch_generic(L.kind, L.epi, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg,
-                                     L.lds, b->stream));
-    }
-  }
+  for (auto& L : b->launches) MVD_CHECK(enqueue_pass_launch(L, b->stream));
   return 0;
 }
 
@@ -1396,72 +1550,12 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
       for (size_t si = 0; si < b->stacks.size(); ++si) {
         mvd_session* s = b->stacks[si];
         if (v >= s->nviews) continue;
-        auto& view = s->views[v];
-        const int wm = s->weight_mode == MVD_WEIGHTS_NONE ? kWeightNone
-                       : s->weight_mode == MVD_WEIGHTS_VIRTUAL ? kWeightVirtual
-                                                               : kWeightArray;
-        int off = 0;
-        for (size_t bi = 0; bi < view.blocks.size(); ++bi) {
-          const ConvGeom& g = view.blocks[bi];
-          PassDesc d{};
-          d.sp = view.plan;
-          d.g = g;
-          d.spec = b->spec[si][bi].as<float2>();
-          d.src = conv == 0 ? s->d_psi.as<float>() : s->d_ratio.as<float>();
-          d.gather = conv == 0 ? kGatherMirror : kGatherOne;
-          d.kspec = conv == 0 ? view.k1hat.as<float2>() : view.k2hat.as<float2>();
-          d.scale = 1.f;
-          const int fused = xy_fused(*view.plan) ? 1 : 0;
-          PassDesc x = d;
-          x.kind = kPassXForward;
-          x.xy = fused;
-          pass[0].push_back(x);
-          PassDesc yf = d;
-          yf.kind = kPassYForward;
-          yf.zlo = 0;
-          yf.zhi = view.plan->q.z;
-          yf.ylo = 0;
-          yf.yhi = view.plan->q.y;
-          yf.dir = -1.f;
-          if (!fused) pass[1].push_back(yf);
-          PassDesc z = d;
-          z.kind = kPassZConv;
-          z.zlo = g.eff_local.z;
-          z.zhi = g.eff_local.z + g.eff_size.z;
-          z.dir = -1.f;
-          pass[2].push_back(z);
-          PassDesc yi = d;
-          yi.kind = kPassYInverse;
-          yi.zlo = g.eff_local.z;
-          yi.zhi = g.eff_local.z + g.eff_size.z;
-          yi.ylo = g.eff_local.y;
-          yi.yhi = g.eff_local.y + g.eff_size.y;
-          yi.dir = 1.f;
-          if (!fused) pass[3].push_back(yi);
-          PassDesc xi = d;
-          xi.kind = kPassXInverse;
-          xi.xy = fused;
-          if (conv == 0) {
-            xi.e.mode = kEpiQuotient;
-            xi.e.dst = s->d_ratio.as<float>();
-            xi.e.img = s->d_views.as<float>() + v * s->n;
-          } else {
-            xi.e.mode = kEpiUpdate;
-            xi.e.dst = s->d_psi.as<float>();
-            xi.e.weight_mode = wm;
-            xi.e.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
-            xi.e.wsum = s->d_wsum.as<float>();
-            xi.e.osem = s->osem;
-            xi.e.lambda = s->lambda;
-            xi.e.part_sum = s->d_part_sum.as<double>() + off;
-            xi.e.part_max = s->d_part_max.as<float>() + off;
-            off += conv_inverse_workgroups(*view.plan, g);
-          }
-          pass[4].push_back(xi);
-        }
+        std::vector<float2*> spec;
+        for (auto& buf : b->spec[si]) spec.push_back(buf.as<float2>());
+        conv_pass_descs(s, v, conv, spec, pass);
       }
       for (int p = 0; p < 5; ++p) {
-        int rc = add_pass_launches(b, pass[p]);
+        int rc = build_pass_launches(pass[p], &b->launches);
         if (rc) return rc;
       }
     }
@@ -1475,8 +1569,9 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
       red.push_back({s->d_part_sum.as<double>(), s->d_part_max.as<float>(), n,
                      b->stats_iter.as<double>() + b->stats_off[si] + 2 * v});
     }
-    mvd_batch::Launch L;
+    PassLaunch L;
     L.reduce = true;
+    L.timing = kTReduce;
     L.nitems = (int)red.size();
     MVD_CHECK(L.items.ensure(sizeof(ReduceItem) * red.size()));
     MVD_CHECK(hipMemcpy(L.items.p, red.data(), sizeof(ReduceItem) * red.size(), hipMemcpyHostToDevice));
