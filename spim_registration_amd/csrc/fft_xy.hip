@@ -35,7 +35,7 @@ constexpr int kXYMP = kXYM + 1;  // LDS row pitch of an x half (complex)
 constexpr int kXYLP = 65;        // LDS pitch of a y round: 64 columns + 1
 constexpr int kXYHalf = 128;     // rows per x half
 constexpr int kXYLds = kXYN * kXYLP;
-constexpr int kXYEpiRows = 16;   // epilogue rows whose loads are issued together
+constexpr int kXYEpiRows = 8;    // epilogue rows whose loads are issued together
 static_assert(kXYHalf * kXYMP <= kXYLds, "x half must fit the y-round buffer");
 
 using SX = Spec<kXYM>;  // 16 x 8: 16 values per thread, 8 threads per row
@@ -59,17 +59,77 @@ __device__ __forceinline__ void xy_fft(float2 (&v)[16], float2* lds, int jt, con
   __syncthreads();
 }
 
+// Two 256-point column transforms per thread (columns a and b, same row set,
+// so the same twiddles): first radix-16 pass on both, LDS exchange of a then
+// of b, twiddles loaded once, second pass on both.  Barriers before each
+// exchange and at the end.
+template <class Addr>
+__device__ __forceinline__ void xy_fft_pair(float2 (&a)[16], float2 (&b)[16], float2* lds, int jt, const float2* tw,
+                                            float dir, Addr addr) {
+  static_assert(SY::R0 == 16 && SY::R1 == 16, "256 = 16 x 16");
+  butterfly<16>(a, dir);
+  __builtin_amdgcn_sched_barrier(0);
+  butterfly<16>(b, dir);
+  __syncthreads();
+  store_pass<kXYN, 16, 16, 1>(a, lds, jt, addr);
+  __syncthreads();
+  load_pass<kXYN, 16, 16>(a, lds, jt, addr);
+  __syncthreads();
+  store_pass<kXYN, 16, 16, 1>(b, lds, jt, addr);
+  __syncthreads();
+  load_pass<kXYN, 16, 16>(b, lds, jt, addr);
+  // second pass: butterfly j = jt reads x[jt + 16 t] with twiddle W^(t jt)
+#pragma unroll
+  for (int t = 1; t < 16; ++t) {
+    float2 w = tw[t * jt];
+    w.y *= -dir;
+    a[t] = cmul(a[t], w);
+    b[t] = cmul(b[t], w);
+    // one twiddle live at a time: the plane already fills the register file
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  butterfly<16>(a, dir);
+  __builtin_amdgcn_sched_barrier(0);  // keep the two butterflies apart (register pressure)
+  butterfly<16>(b, dir);
+  __syncthreads();
+}
+
 // 128-point transform of the rows of an x half, in place in LDS.
 __device__ __forceinline__ void xy_rows_fft(float2* lds, const float2* tw, float dir) {
+  static_assert(SX::R0 == 16 && SX::R1 == 8, "128 = 16 x 8");
   const int line = threadIdx.x & (kXYHalf - 1), jt = threadIdx.x >> 7;
   auto addr = [line](int i) { return line * kXYMP + i; };
   float2 v[16];
-  load_pass<kXYM, 16, SX::R0>(v, lds, jt, addr);
-  xy_fft<kXYM>(v, lds, jt, tw, 2, dir, addr);
+  load_pass<kXYM, 16, 16>(v, lds, jt, addr);
+  butterfly<16>(v, dir);
+  __syncthreads();
+  store_pass<kXYM, 16, 16, 1>(v, lds, jt, addr);
+  __syncthreads();
+  load_pass<kXYM, 16, 8>(v, lds, jt, addr);
+  // radix-8 pass, butterflies k = jt and k = jt + 8 with twiddles W128^(t k).
+  // Those of k + 8 follow from those of k by the constant W128^(8 t) =
+  // exp(-i pi t / 8), so only seven twiddles are live at a time.
+  constexpr float kC[8] = {1.f, 0.923879532511286756f, 0.707106781186547524f, 0.382683432365089772f,
+                           0.f, -0.382683432365089772f, -0.707106781186547524f, -0.923879532511286756f};
+  constexpr float kS[8] = {0.f, 0.382683432365089772f, 0.707106781186547524f, 0.923879532511286756f,
+                           1.f, 0.923879532511286756f, 0.707106781186547524f, 0.382683432365089772f};
+  float2 w[8];
 #pragma unroll
-  for (int b = 0; b < 16 / SX::R1; ++b)
+  for (int t = 1; t < 8; ++t) w[t] = tw[2 * t * jt];  // exp(-2 pi i t jt / 128)
 #pragma unroll
-    for (int t = 0; t < SX::R1; ++t) lds[addr(elem_index<kXYM, 16, SX::R1>(jt, b, t))] = v[b * SX::R1 + t];
+  for (int t = 1; t < 8; ++t) v[t] = cmul(v[t], make_float2(w[t].x, -dir * w[t].y));
+  butterfly<8>(v, dir);
+#pragma unroll
+  for (int t = 1; t < 8; ++t) {
+    const float2 w8 = cmul(w[t], make_float2(kC[t], -kS[t]));
+    v[8 + t] = cmul(v[8 + t], make_float2(w8.x, -dir * w8.y));
+  }
+  butterfly<8>(v + 8, dir);
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) lds[addr(elem_index<kXYM, 16, 8>(jt, b, t))] = v[b * 8 + t];
 }
 
 }  // namespace
@@ -232,8 +292,48 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       c0[t] = make_float2(c0[t].x - b.y, c0[t].y + b.x);
     }
   }
-  xy_fft<kXYN>(c0, lds, jy, tw, 1, 1.f, yaddr);
-  xy_fft<kXYN>(c1, lds, jy, tw, 1, 1.f, yaddr);
+  // y transforms of both column sets.  After the two LDS exchanges the y tile
+  // is free, so the last radix pass of each set writes its rows of the first
+  // half straight into the x tile: at most 48 plane values per thread are live.
+  auto put_rows = [&](const float2 (&c)[16], int col, int h) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      float2* row = lds + (jy + 16 * t) * kXYMP;
+      const float2 a = c[t + 8 * h];
+      if (col == 0 && cl == 0) {  // DC + i Nyquist
+        row[0] = make_float2(a.x, 0.f);
+        row[kXYM] = make_float2(a.y, 0.f);
+      } else {
+        row[col + cl] = a;
+      }
+    }
+  };
+  auto last_pass = [&](float2 (&c)[16]) {
+#pragma unroll
+    for (int t = 1; t < 16; ++t) {
+      float2 w = tw[t * jy];  // butterfly jy of the second radix-16 pass: W256^(t jy)
+      w.y = -w.y;             // inverse direction
+      c[t] = cmul(c[t], w);
+      __builtin_amdgcn_sched_barrier(0);  // one twiddle live at a time
+    }
+    butterfly<16>(c, 1.f);
+  };
+  butterfly<16>(c0, 1.f);
+  __builtin_amdgcn_sched_barrier(0);
+  butterfly<16>(c1, 1.f);
+  __syncthreads();
+  store_pass<kXYN, 16, 16, 1>(c0, lds, jy, yaddr);
+  __syncthreads();
+  load_pass<kXYN, 16, 16>(c0, lds, jy, yaddr);
+  __syncthreads();
+  store_pass<kXYN, 16, 16, 1>(c1, lds, jy, yaddr);
+  __syncthreads();
+  load_pass<kXYN, 16, 16>(c1, lds, jy, yaddr);
+  __syncthreads();
+  last_pass(c0);
+  put_rows(c0, 0, 0);
+  last_pass(c1);
+  put_rows(c1, 64, 0);
   double my_sum = 0.0;
   float my_max = -1.f;
   const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
@@ -303,21 +403,12 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   float p0[R], p1[R];  // first batch of a half, loaded ahead of the transform where registers allow
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    __syncthreads();  // earlier readers of lds are done
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      float2* row = lds + (jy + 16 * t) * kXYMP;
-      const float2 a = c0[t + 8 * h];
-      if (cl == 0) {
-        row[0] = make_float2(a.x, 0.f);
-        row[kXYM] = make_float2(a.y, 0.f);
-      } else {
-        row[cl] = a;
-      }
-      row[64 + cl] = c1[t + 8 * h];
+    if (h == 1) {
+      __syncthreads();  // the first half's readers of lds are done
+      put_rows(c0, 0, 1);
+      put_rows(c1, 64, 1);
+      load_batch(1, 0, p0, p1);  // no plane values are held in registers any more
     }
-    // the second half holds no y columns in registers any more
-    if (h == 1) load_batch(1, 0, p0, p1);
     __syncthreads();
     // pack X[k], X[M - k] into Z[k] = E + i O
     for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
