@@ -326,9 +326,10 @@ def main():
 
     if args.copy_probe:
         src = torch.empty(64 * 2**20, dtype=torch.float32, device="cuda").fill_(1.0)
+        flush = torch.empty(256 * 2**20, dtype=torch.float32, device="cuda").fill_(0.0)  # evicts src from the caches
         dst = src * 1.0  # one elementwise kernel: reads 256 MiB, writes 256 MiB
         torch.cuda.synchronize()
-        del src, dst
+        del src, dst, flush
     runner = d
     if args.batched:
         from spim_registration_amd import MVDeconBatch
