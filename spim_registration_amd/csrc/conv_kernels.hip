@@ -43,16 +43,16 @@ __device__ __forceinline__ float gather_value(const XFwdParams& P, int x, int y,
     if (jy >= g.q.y) jy -= g.q.y;
     if (jz >= g.q.z) jz -= g.q.z;
     if (jx < g.src.x && jy < g.src.y && jz < g.src.z)
-      return P.src[jx + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz)];
+      return gmem(P.src)[jx + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz)];
     return 0.f;
   }
   const int gx = g.off.x + x, gy = g.off.y + y, gz = g.off.z + z;
   const bool inside = gx >= 0 && gx < g.src.x && gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
-  if (inside) return P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+  if (inside) return gmem(P.src)[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
   if (P.gather == kGatherOne) return 1.f;
   if (P.gather == kGatherZero) return 0.f;
   const int mx = mirror_single(gx, g.src.x), my = mirror_single(gy, g.src.y), mz = mirror_single(gz, g.src.z);
-  return P.src[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
+  return gmem(P.src)[mx + (int64_t)g.src.x * (my + (int64_t)g.src.y * mz)];
 }
 
 __device__ __forceinline__ void x_forward_body(const XFwdParams& P, int bx, int by) {
@@ -61,7 +61,7 @@ __device__ __forceinline__ void x_forward_body(const XFwdParams& P, int bx, int 
   const int B = P.lines, lp = B + 1;
   float2* tw = smem;
   float2* buf = smem + qx;
-  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int nrows = P.g.q.y * P.g.q.z;
   const int row0 = bx * B;
   float* fbuf = reinterpret_cast<float*>(buf);
@@ -85,7 +85,7 @@ __device__ __forceinline__ void x_forward_body(const XFwdParams& P, int bx, int 
     if (row >= nrows) continue;
     const int km = (M - k) % M;
     const float2 a = buf[k * lp + l], b = buf[km * lp + l];
-    float2* out = P.spec + (int64_t)row * P.pitch;
+    const auto out = gmem(P.spec) + (int64_t)row * P.pitch;
     {
       const float2 bc = cconj(b);
       const float2 E = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
@@ -107,7 +107,8 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_batch(const XFwdParams* 
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  x_forward_body(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  x_forward_body(item_params, bx, by);
 }
 
 struct YParams {
@@ -123,10 +124,10 @@ __device__ __forceinline__ void y_body(const YParams& P, int bx, int by) {
   const int B = P.lines, lp = B + 1, qy = P.qy;
   float2* tw = smem;
   float2* buf = smem + qy;
-  for (int i = threadIdx.x; i < qy; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < qy; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int c0 = bx * B;
   const int z = P.zlo + by;
-  float2* base = P.spec + (int64_t)z * qy * P.pitch + c0;
+  const auto base = gmem(P.spec) + (int64_t)z * qy * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qy * B; idx += kThreads) {
     const int y = idx / B, b = idx - y * B;
     buf[y * lp + b] = (c0 + b < P.hx) ? base[(int64_t)y * P.pitch + b] : make_float2(0.f, 0.f);
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(kThreads) void k_y_batch(const YParams* items, cons
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  y_body(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  y_body(item_params, bx, by);
 }
 
 struct ZParams {
@@ -163,33 +165,34 @@ __device__ __forceinline__ void z_body(const ZParams& P, int bx, int by) {
   const int B = P.lines, lp = B + 1, qz = P.qz;
   float2* tw = smem;
   float2* buf = smem + qz;
-  for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int c0 = bx * B;
   const int y = by;
   const int64_t plane = (int64_t)P.qy * P.pitch;
   const int64_t col = (int64_t)y * P.pitch + c0;
   for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
     const int z = idx / B, b = idx - z * B;
-    buf[z * lp + b] = (c0 + b < P.hx) ? P.spec[z * plane + col + b] : make_float2(0.f, 0.f);
+    buf[z * lp + b] = make_float2(0.f, 0.f);
+    if (c0 + b < P.hx) buf[z * lp + b] = gmem(P.spec)[z * plane + col + b];
   }
   __syncthreads();
   lds_fft(buf, lp, B, P.plan, tw, 1, P.kspec ? -1.f : P.dir);
   if (P.kspec) {
     for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
       const int z = idx / B, b = idx - z * B;
-      if (c0 + b < P.hx) buf[z * lp + b] = cmul(buf[z * lp + b], P.kspec[z * plane + col + b]);
+      if (c0 + b < P.hx) buf[z * lp + b] = cmul(buf[z * lp + b], gmem(P.kspec)[z * plane + col + b]);
     }
     __syncthreads();
     lds_fft(buf, lp, B, P.plan, tw, 1, 1.f);
     const int nrow = P.zhi - P.zlo;
     for (int idx = threadIdx.x; idx < nrow * B; idx += kThreads) {
       const int z = P.zlo + idx / B, b = idx % B;
-      if (c0 + b < P.hx) P.spec[z * plane + col + b] = buf[z * lp + b];
+      if (c0 + b < P.hx) gmem(P.spec)[z * plane + col + b] = buf[z * lp + b];
     }
   } else {
     for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
       const int z = idx / B, b = idx - z * B;
-      if (c0 + b < P.hx) P.spec[z * plane + col + b] = cscale(buf[z * lp + b], P.scale);
+      if (c0 + b < P.hx) gmem(P.spec)[z * plane + col + b] = cscale(buf[z * lp + b], P.scale);
     }
   }
 }
@@ -200,7 +203,8 @@ __global__ __launch_bounds__(kThreads) void k_z_batch(const ZParams* items, cons
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  z_body(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  z_body(item_params, bx, by);
 }
 
 struct XInvParams {
@@ -221,7 +225,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
   const int B = P.lines, lp = B + 1;
   float2* tw = smem;
   float2* buf = smem + qx;
-  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < qx; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
   const int row0 = bx * B;
@@ -232,7 +236,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) {
       const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-      v = P.spec[((int64_t)z * g.q.y + y) * P.pitch + k];
+      v = gmem(P.spec)[((int64_t)z * g.q.y + y) * P.pitch + k];
     }
     buf[k * lp + l] = v;
   }
@@ -266,29 +270,29 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
     const int64_t gi = (int64_t)(g.eff_offset.x + xo) +
                        (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
     if (e.mode == kEpiStore) {
-      e.dst[gi] = v;
+      gmem(e.dst)[gi] = v;
     } else if (e.mode == kEpiResidual) {
-      const double w = e.weight_mode == kWeightArray ? (double)e.weight[gi] : 1.0;
-      const double b = (double)v, y = (double)e.img[gi];
+      const double w = e.weight_mode == kWeightArray ? (double)gmem(e.weight)[gi] : 1.0;
+      const double b = (double)v, y = (double)gmem(e.img)[gi];
       if (b > 0) {
         my_sum += w * (b - y * log(b));
-        e.dst[gi] = (float)(w * (1.0 - y / b));
+        gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
       } else {
-        e.dst[gi] = 0.f;
+        gmem(e.dst)[gi] = 0.f;
       }
     } else if (e.mode == kEpiQuotient) {
-      e.dst[gi] = rl_quotient(v, e.img[gi]);
+      gmem(e.dst)[gi] = rl_quotient(v, gmem(e.img)[gi]);
     } else {
       float w = 1.f;
       if (e.weight_mode == kWeightArray) {
-        w = e.weight[gi];
+        w = gmem(e.weight)[gi];
       } else if (e.weight_mode == kWeightVirtual) {
-        const double r = (double)e.weight[gi] / (double)e.wsum[gi];
+        const double r = (double)gmem(e.weight)[gi] / (double)gmem(e.wsum)[gi];
         w = (float)fmin(1.0, r * e.osem);
       }
-      const float last = e.dst[gi];
+      const float last = gmem(e.dst)[gi];
       const float next = rl_next_value(last, v, w, e.lambda);
-      e.dst[gi] = next;
+      gmem(e.dst)[gi] = next;
       const float change = fabsf(next - last);
       my_sum += (double)change;
       my_max = fmaxf(my_max, change);
@@ -312,8 +316,8 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
         s += red_sum[i];
         m = fmaxf(m, red_max[i]);
       }
-      e.part_sum[bx] = s;
-      e.part_max[bx] = m;
+      gmem(e.part_sum)[bx] = s;
+      gmem(e.part_max)[bx] = m;
     }
   }
 }
@@ -324,7 +328,8 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_batch(const XInvParams* 
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  x_inverse_body(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  x_inverse_body(item_params, bx, by);
 }
 
 // ---------------------------------------------------------------------------

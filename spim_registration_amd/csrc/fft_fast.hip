@@ -27,11 +27,11 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int col = bx * L + line;
   const bool valid = col < P.hx;
   const int z = P.zlo + by;
-  float2* base = P.spec + (int64_t)z * N * P.pitch + col;
+  const auto base = gmem(P.spec) + (int64_t)z * N * P.pitch + col;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kThreads) void k_y_fast_batch(const YFastParams* it
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  y_fast_body<N>(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  y_fast_body<N>(item_params, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -86,7 +87,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int col = bx * L + line;
   const bool valid = col < P.hx;
   const int64_t plane = (int64_t)P.qy * P.pitch;
@@ -97,7 +98,8 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
     for (int t = 0; t < S::R0; ++t) {
       const int z = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = valid ? P.spec[z * plane + off] : make_float2(0.f, 0.f);
+      v[b * S::R0 + t] = make_float2(0.f, 0.f);
+      if (valid) v[b * S::R0 + t] = gmem(P.spec)[z * plane + off];
     }
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
@@ -108,7 +110,8 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int z = elem_index<N, E, RL>(jt, b, t);
-        const float2 k = valid ? P.kspec[z * plane + off] : make_float2(0.f, 0.f);
+        float2 k = make_float2(0.f, 0.f);
+        if (valid) k = gmem(P.kspec)[z * plane + off];
         v[b * RL + t] = cmul(v[b * RL + t], k);
       }
     fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
@@ -118,7 +121,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
         for (int t = 0; t < S::R0; ++t) {
           const int z = elem_index<N, E, S::R0>(jt, b, t);
-          if (z >= P.zlo && z < P.zhi) P.spec[z * plane + off] = v[b * S::R0 + t];
+          if (z >= P.zlo && z < P.zhi) gmem(P.spec)[z * plane + off] = v[b * S::R0 + t];
         }
     }
   } else if (valid) {
@@ -127,7 +130,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int z = elem_index<N, E, RL>(jt, b, t);
-        P.spec[z * plane + off] = cscale(v[b * RL + t], P.scale);
+        gmem(P.spec)[z * plane + off] = cscale(v[b * RL + t], P.scale);
       }
   }
 }
@@ -142,7 +145,8 @@ __global__ __launch_bounds__(kThreads) void k_z_fast_batch(const ZFastParams* it
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  z_fast_body<N>(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  z_fast_body<N>(item_params, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,7 +175,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
   __shared__ float2 buf[L * MP];
   float* fbuf = reinterpret_cast<float*>(buf);
   const ConvGeom& g = P.g;
-  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int nrows = g.q.y * g.q.z;
   const int row0 = bx * L;
   const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
@@ -185,7 +189,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
     const int row = row0 + l;
     const int y = row % g.q.y, z = row / g.q.y;
     const bool live = row < nrows && y < g.win.y && z < g.win.z;
-    const float* srow = nullptr;
+    decltype(gmem(P.src)) srow = nullptr;
     bool row_inside = false;  // the whole (y, z) row is inside the source
     if (live) {
       if (MODE == kGatherKernelWrap) {
@@ -193,7 +197,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
         if (jy >= g.q.y) jy -= g.q.y;
         if (jz >= g.q.z) jz -= g.q.z;
         row_inside = jy < g.src.y && jz < g.src.z;
-        srow = P.src + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz);
+        srow = gmem(P.src) + (int64_t)g.src.x * (jy + (int64_t)g.src.y * jz);
       } else {
         int gy = g.off.y + y, gz = g.off.z + z;
         row_inside = gy >= 0 && gy < g.src.y && gz >= 0 && gz < g.src.z;
@@ -202,7 +206,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
           gz = mirror_single(gz, g.src.z);
           row_inside = true;  // the mirrored row is a source row
         }
-        srow = P.src + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz);
+        srow = gmem(P.src) + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz);
       }
     }
 #pragma unroll
@@ -256,7 +260,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
     const int row = row0 + l;
     if (row >= nrows) continue;
     const float2 a = buf[l * MP + k], b = buf[l * MP + ((M - k) & (M - 1))];
-    float2* out = P.spec + (int64_t)row * P.pitch;
+    const auto out = gmem(P.spec) + (int64_t)row * P.pitch;
     {
       const float2 bc = cconj(b);
       const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
@@ -280,7 +284,8 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_fast_batch(const XFwdFas
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  x_forward_fast_body<M, MODE>(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  x_forward_fast_body<M, MODE>(item_params, bx, by);
 }
 
 struct XInvFastParams {
@@ -305,7 +310,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   __shared__ double red_sum[kThreads / 64];
   __shared__ float red_max[kThreads / 64];
   const ConvGeom& g = P.g;
-  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
   const int row0 = bx * L;
@@ -317,13 +322,16 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int row = row0 + r * RPI_C + lsub;
-      const float2* srow = nullptr;
+      decltype(gmem(P.spec)) srow = nullptr;
       if (row < nrows) {
         const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-        srow = P.spec + ((int64_t)z * g.q.y + y) * P.pitch;
+        srow = gmem(P.spec) + ((int64_t)z * g.q.y + y) * P.pitch;
       }
 #pragma unroll
-      for (int e = 0; e < EPT_C; ++e) vals[r][e] = srow ? srow[k0 + e * kThreads] : make_float2(0.f, 0.f);
+      for (int e = 0; e < EPT_C; ++e) {
+        vals[r][e] = make_float2(0.f, 0.f);
+        if (srow) vals[r][e] = srow[k0 + e * kThreads];
+      }
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r)
@@ -335,7 +343,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       float2 val = make_float2(0.f, 0.f);
       if (row < nrows) {
         const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-        val = P.spec[((int64_t)z * g.q.y + y) * P.pitch + M];
+        val = gmem(P.spec)[((int64_t)z * g.q.y + y) * P.pitch + M];
       }
       buf[l * MP + M] = val;
     }
@@ -395,11 +403,11 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       in1[r][ee] = 1.f;
       if (live) {
         if (EPI == kEpiQuotient || EPI == kEpiResidual) {
-          in0[r][ee] = e.img[gi];
-          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[r][ee] = e.weight[gi];
+          in0[r][ee] = gmem(e.img)[gi];
+          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[r][ee] = gmem(e.weight)[gi];
         } else if (EPI == kEpiUpdate) {
-          in0[r][ee] = e.dst[gi];
-          if (e.weight_mode != kWeightNone) in1[r][ee] = e.weight[gi];
+          in0[r][ee] = gmem(e.dst)[gi];
+          if (e.weight_mode != kWeightNone) in1[r][ee] = gmem(e.weight)[gi];
         }
       }
     }
@@ -415,26 +423,26 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       const int xo = x0 + ee * kThreads;
       const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
       if (EPI == kEpiStore) {
-        e.dst[gi] = val;
+        gmem(e.dst)[gi] = val;
       } else if (EPI == kEpiQuotient) {
-        e.dst[gi] = rl_quotient(val, in0[r][ee]);
+        gmem(e.dst)[gi] = rl_quotient(val, in0[r][ee]);
       } else if (EPI == kEpiResidual) {
         const double w = (double)in1[r][ee], b = (double)val, y = (double)in0[r][ee];
         if (b > 0) {
           my_sum += w * (b - y * log(b));
-          e.dst[gi] = (float)(w * (1.0 - y / b));
+          gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
         } else {
-          e.dst[gi] = 0.f;
+          gmem(e.dst)[gi] = 0.f;
         }
       } else {
         float w = in1[r][ee];
         if (e.weight_mode == kWeightVirtual) {
-          const double rr = (double)in1[r][ee] / (double)e.wsum[gi];
+          const double rr = (double)in1[r][ee] / (double)gmem(e.wsum)[gi];
           w = (float)fmin(1.0, rr * e.osem);
         }
         const float last = in0[r][ee];
         const float next = rl_next_value(last, val, w, e.lambda);
-        e.dst[gi] = next;
+        gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
         my_max = fmaxf(my_max, change);
@@ -459,8 +467,8 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
         s += red_sum[i];
         m = fmaxf(m, red_max[i]);
       }
-      e.part_sum[bx] = s;
-      e.part_max[bx] = m;
+      gmem(e.part_sum)[bx] = s;
+      gmem(e.part_max)[bx] = m;
     }
   }
 }
@@ -475,7 +483,8 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast_batch(const XInvFas
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  x_inverse_fast_body<M, EPI>(items[item], bx, by);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  x_inverse_fast_body<M, EPI>(item_params, bx, by);
 }
 
 // ---------------------------------------------------------------------------

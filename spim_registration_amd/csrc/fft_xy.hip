@@ -44,6 +44,14 @@ static_assert(SX::E == 16 && SY::E == 16 && SX::R2 == 1 && SY::R2 == 1, "fused x
 static_assert(kXYThreads / (kXYM / SX::E) == kXYHalf, "one x row per 8 threads");
 static_assert(kXYThreads / (kXYN / SY::E) == 64, "64 y columns per round");
 
+// Returns v unchanged but opaque to the optimiser, so that index arithmetic
+// derived from it is recomputed in each phase instead of being kept live (and
+// spilled) across the whole kernel: the plane already fills the register file.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Full transform of 16 values per thread; outputs in the order of the last
 // radix.  Barriers before the LDS exchange and after it.
 template <int N, class Addr>
@@ -97,7 +105,8 @@ __device__ __forceinline__ void xy_fft_pair(float2 (&a)[16], float2 (&b)[16], fl
 // 128-point transform of the rows of an x half, in place in LDS.
 __device__ __forceinline__ void xy_rows_fft(float2* lds, const float2* tw, float dir) {
   static_assert(SX::R0 == 16 && SX::R1 == 8, "128 = 16 x 8");
-  const int line = threadIdx.x & (kXYHalf - 1), jt = threadIdx.x >> 7;
+  const int lane = opaque(threadIdx.x);
+  const int line = lane & (kXYHalf - 1), jt = lane >> 7;
   auto addr = [line](int i) { return line * kXYMP + i; };
   float2 v[16];
   load_pass<kXYM, 16, 16>(v, lds, jt, addr);
@@ -150,7 +159,7 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
   const int tid = threadIdx.x;
   const ConvGeom& g = P.g;
   const int z = bx;
-  if (tid < kXYN) tw[tid] = P.tw[tid];
+  if (tid < kXYN) tw[tid] = gmem(P.tw)[tid];
   float* fbuf = reinterpret_cast<float*>(lds);
   // y columns of this thread: cl and 64 + cl, rows jy + 16 t
   const int cl = tid & 63, jy = tid >> 6;
@@ -178,9 +187,9 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
         const bool y_in = gy >= 0 && gy < g.src.y;
         if (MODE == kGatherMirror) {
           if (!y_in) gy = mirror_single(gy, g.src.y);
-          val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+          val = gmem(P.src)[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
         } else if (z_in && y_in && x_in) {
-          val = P.src[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+          val = gmem(P.src)[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
         } else {
           val = MODE == kGatherOne ? 1.f : 0.f;
         }
@@ -201,7 +210,7 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     xy_rows_fft(lds, tw, -1.f);
     __syncthreads();
     // unpack Z = FFT(even + i odd) into X[k] and X[M - k], in place
-    for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
+    for (int idx = opaque(threadIdx.x); idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
       const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
       float2* row = lds + l * kXYMP;
       const float2 a = row[k], b = row[(kXYM - k) & (kXYM - 1)];
@@ -223,7 +232,7 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     }
   }
   // y transforms, 64 columns per round
-  float2* out = P.spec + (int64_t)z * kXYN * P.pitch;
+  const auto out = gmem(P.spec) + (int64_t)z * kXYN * P.pitch;
   auto yaddr = [cl](int i) { return i * kXYLP + cl; };
   xy_fft<kXYN>(c0, lds, jy, tw, 1, -1.f, yaddr);
 #pragma unroll
@@ -272,9 +281,9 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const EpilogueArgs& e = P.e;
   const int zo = bx;                 // output plane inside the effective box
   const int z = g.eff_local.z + zo;  // plane of the FFT box
-  if (tid < kXYN) tw[tid] = P.tw[tid];
+  if (tid < kXYN) tw[tid] = gmem(P.tw)[tid];
   const int cl = tid & 63, jy = tid >> 6;
-  const float2* in = P.spec + (int64_t)z * kXYN * P.pitch;
+  const auto in = gmem(P.spec) + (int64_t)z * kXYN * P.pitch;
   auto yaddr = [cl](int i) { return i * kXYLP + cl; };
   float2 c0[16], c1[16];
 #pragma unroll
@@ -296,6 +305,8 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   // is free, so the last radix pass of each set writes its rows of the first
   // half straight into the x tile: at most 48 plane values per thread are live.
   auto put_rows = [&](const float2 (&c)[16], int col, int h) {
+    const int lane = opaque(threadIdx.x);
+    const int cl = lane & 63, jy = lane >> 6;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       float2* row = lds + (jy + 16 * t) * kXYMP;
@@ -339,61 +350,65 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
   const int nx = g.eff_size.x;
   const float* fbuf = reinterpret_cast<const float*>(lds);
-  const int x = tid & 255, rsub = tid >> 8;
-  const bool x_live = x < nx;
   constexpr int R = kXYEpiRows;
-  auto out_index = [&](int y, int zo_) -> int64_t {
+  auto out_index = [&](int x, int y, int zo_) -> int64_t {
     return (int64_t)(g.eff_offset.x + x) +
            (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + y - ylo) + (int64_t)g.dst.y * (g.eff_offset.z + zo_));
   };
-  // inputs of the epilogue for rows rsub + 4 (c R + i) of half h
+  // inputs of the epilogue for rows rsub + 4 (c R + i) of half h; lanes run along x
   auto load_batch = [&](int h, int c, float (&in0)[R], float (&in1)[R]) {
+    const int lane = opaque(threadIdx.x);
+    const int x = lane & 255, rsub = lane >> 8;
+    const bool x_live = x < nx;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int y = h * kXYHalf + rsub + 4 * (c * R + i);
       in0[i] = 0.f;
       in1[i] = 1.f;
       if (x_live && y >= ylo && y < yhi) {
-        const int64_t gi = out_index(y, zo);
+        const int64_t gi = out_index(x, y, zo);
         if (EPI == kEpiQuotient || EPI == kEpiResidual) {
-          in0[i] = e.img[gi];
-          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = e.weight[gi];
+          in0[i] = gmem(e.img)[gi];
+          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = gmem(e.weight)[gi];
         } else if (EPI == kEpiUpdate) {
-          in0[i] = e.dst[gi];
-          if (e.weight_mode != kWeightNone) in1[i] = e.weight[gi];
+          in0[i] = gmem(e.dst)[gi];
+          if (e.weight_mode != kWeightNone) in1[i] = gmem(e.weight)[gi];
         }
       }
     }
   };
   auto run_batch = [&](int h, int c, const float (&in0)[R], const float (&in1)[R]) {
+    const int lane = opaque(threadIdx.x);
+    const int x = lane & 255, rsub = lane >> 8;
+    const bool x_live = x < nx;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int yl = rsub + 4 * (c * R + i);
       const int y = h * kXYHalf + yl;
       if (!x_live || y < ylo || y >= yhi) continue;
-      const int64_t gi = out_index(y, zo);
+      const int64_t gi = out_index(x, y, zo);
       const float val = fbuf[yl * 2 * kXYMP + g.eff_local.x + x];
       if (EPI == kEpiStore) {
-        e.dst[gi] = val;
+        gmem(e.dst)[gi] = val;
       } else if (EPI == kEpiQuotient) {
-        e.dst[gi] = rl_quotient(val, in0[i]);
+        gmem(e.dst)[gi] = rl_quotient(val, in0[i]);
       } else if (EPI == kEpiResidual) {
         const double w = (double)in1[i], b = (double)val, yv = (double)in0[i];
         if (b > 0) {
           my_sum += w * (b - yv * log(b));
-          e.dst[gi] = (float)(w * (1.0 - yv / b));
+          gmem(e.dst)[gi] = (float)(w * (1.0 - yv / b));
         } else {
-          e.dst[gi] = 0.f;
+          gmem(e.dst)[gi] = 0.f;
         }
       } else {
         float w = in1[i];
         if (e.weight_mode == kWeightVirtual) {
-          const double r = (double)in1[i] / (double)e.wsum[gi];
+          const double r = (double)in1[i] / (double)gmem(e.wsum)[gi];
           w = (float)fmin(1.0, r * e.osem);
         }
         const float last = in0[i];
         const float next = rl_next_value(last, val, w, e.lambda);
-        e.dst[gi] = next;
+        gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
         my_max = fmaxf(my_max, change);
@@ -411,7 +426,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     }
     __syncthreads();
     // pack X[k], X[M - k] into Z[k] = E + i O
-    for (int idx = tid; idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
+    for (int idx = opaque(threadIdx.x); idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
       const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
       float2* row = lds + l * kXYMP;
       const float2 a = row[k], b = row[kXYM - k];
@@ -453,8 +468,8 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
         s += red_sum[i];
         m = fmaxf(m, red_max[i]);
       }
-      e.part_sum[bx] = s;
-      e.part_max[bx] = m;
+      gmem(e.part_sum)[bx] = s;
+      gmem(e.part_max)[bx] = m;
     }
   }
 }
@@ -468,7 +483,8 @@ __global__ __launch_bounds__(kXYThreads) void k_xy_forward_batch(const XYFwdPara
                                                                 const int* gx, int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  xy_forward_body<MODE>(items[item], bx);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  xy_forward_body<MODE>(item_params, bx);
 }
 template <int EPI>
 __global__ __launch_bounds__(kXYThreads) void k_xy_inverse(XYInvParams P) {
@@ -479,7 +495,8 @@ __global__ __launch_bounds__(kXYThreads) void k_xy_inverse_batch(const XYInvPara
                                                                 const int* gx, int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  xy_inverse_body<EPI>(items[item], bx);
+  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
+  xy_inverse_body<EPI>(item_params, bx);
 }
 
 // ---------------------------------------------------------------------------

@@ -83,6 +83,21 @@ __device__ __forceinline__ void batch_locate(const int* prefix, const int* gx, i
   by = local / gx[lo];
 }
 
+// Global-memory view of a pointer held in a parameter block: accesses through
+// it compile to global_* instead of flat_* instructions.  Flat accesses also
+// count against the LDS wait counter, so every LDS barrier would wait for
+// them and no global load could stay in flight across a barrier.
+// (The host pass of the single-source compile only parses device code; it
+// keeps the generic pointer.)
+template <class T>
+__device__ __forceinline__ auto gmem(T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (__attribute__((address_space(1))) T*)p;
+#else
+  return p;
+#endif
+}
+
 FftPlan make_plan(int n);
 inline int spectrum_pitch(int hx) { return (hx + 15) / 16 * 16; }
 bool fft_size_supported(int n);
