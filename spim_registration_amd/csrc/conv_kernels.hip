@@ -107,8 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_x_forward_batch(const XFwdParams* 
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
-  x_forward_body(item_params, bx, by);
+  x_forward_body(items[item], bx, by);
 }
 
 struct YParams {
@@ -147,8 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_y_batch(const YParams* items, cons
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
-  y_body(item_params, bx, by);
+  y_body(items[item], bx, by);
 }
 
 struct ZParams {
@@ -203,8 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_z_batch(const ZParams* items, cons
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
-  z_body(item_params, bx, by);
+  z_body(items[item], bx, by);
 }
 
 struct XInvParams {
@@ -328,8 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_batch(const XInvParams* 
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
-  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
-  x_inverse_body(item_params, bx, by);
+  x_inverse_body(items[item], bx, by);
 }
 
 // ---------------------------------------------------------------------------

@@ -797,7 +797,15 @@ static int pow2_at_least(int n) {
 }
 
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, Int3> g_tune_cache;
+struct TunedShape {
+  Int3 q;
+  float ms;  // one block convolution
+};
+std::map<std::tuple<int, int, int>, TunedShape> g_tune_cache;  // window -> FFT shape
+struct TunedBlocks {
+  Int3 bs, q;
+};
+std::map<std::tuple<int, int, int, int, int, int>, TunedBlocks> g_block_cache;  // (image, kernel) -> blocks
 
 int time_shape(mvd_session* s, Int3 win, Int3 q, float* ms_out) {
   const ShapePlan* sp = s->plans.get(q);
@@ -843,24 +851,19 @@ int time_shape(mvd_session* s, Int3 win, Int3 q, float* ms_out) {
   return 0;
 }
 
-int choose_fft_shape(mvd_session* s, Int3 win, Int3* q) {
+// Fastest FFT shape for a window among the smallest 5-smooth and power-of-two
+// lengths per axis (all 8 combinations), timed on the GPU and cached.
+int tune_fft_shape(mvd_session* s, Int3 win, Int3* q, float* ms_out) {
   const Int3 smooth = fft_shape_for(win);
-  if (s->fft_policy == MVD_FFT_SMOOTH) {
-    *q = smooth;
-    return 0;
-  }
   auto p2 = [](int w, int sm) { const int p = pow2_at_least(w); return p <= 1024 ? p : sm; };
   const Int3 pow2 = {p2(win.x, smooth.x), p2(win.y, smooth.y), p2(win.z, smooth.z)};
-  if (s->fft_policy == MVD_FFT_POW2) {
-    *q = pow2;
-    return 0;
-  }
   const auto key = std::make_tuple(win.x, win.y, win.z);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      *q = it->second;
+      *q = it->second.q;
+      if (ms_out) *ms_out = it->second.ms;
       return 0;
     }
   }
@@ -881,9 +884,70 @@ int choose_fft_shape(mvd_session* s, Int3 win, Int3* q) {
   }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = best;
+    g_tune_cache[key] = {best, best_ms};
   }
   *q = best;
+  if (ms_out) *ms_out = best_ms;
+  return 0;
+}
+
+int choose_fft_shape(mvd_session* s, Int3 win, Int3* q) {
+  const Int3 smooth = fft_shape_for(win);
+  if (s->fft_policy == MVD_FFT_SMOOTH) {
+    *q = smooth;
+    return 0;
+  }
+  if (s->fft_policy == MVD_FFT_POW2) {
+    auto p2 = [](int w, int sm) { const int p = pow2_at_least(w); return p <= 1024 ? p : sm; };
+    *q = {p2(win.x, smooth.x), p2(win.y, smooth.y), p2(win.z, smooth.z)};
+    return 0;
+  }
+  return tune_fft_shape(s, win, q, nullptr);
+}
+
+// Autotuned block shape for a view the caller did not split (one window of
+// img + K - 1 in the reference): the whole window with its best FFT shape, or
+// blocks of 256 x 256 x {128, 256} whose planes take the fused x-y kernels,
+// whichever convolves the image fastest (blocks x time per block).  Blocked
+// and whole-image convolutions are the same operator (each block window
+// carries the kernel halo and the mirror-single extension), so this changes
+// rounding only.
+int choose_blocks(mvd_session* s, Int3 kd, Int3* bs_out, Int3* q_out) {
+  const auto key = std::make_tuple(s->img.x, s->img.y, s->img.z, kd.x, kd.y, kd.z);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_block_cache.find(key);
+    if (it != g_block_cache.end()) {
+      *bs_out = it->second.bs;
+      *q_out = it->second.q;
+      return 0;
+    }
+  }
+  const Int3 whole = {s->img.x + kd.x - 1, s->img.y + kd.y - 1, s->img.z + kd.z - 1};
+  Int3 best_bs = whole, best_q;
+  float t_whole = 0.f;
+  int rc = tune_fft_shape(s, whole, &best_q, &t_whole);
+  if (rc) return rc;
+  double best_ms = t_whole;
+  for (int bz : {128, 256}) {
+    const Int3 b = {256, 256, bz};
+    std::vector<ConvGeom> blocks;
+    if (!divide_into_blocks(s->img, kd, b, b, &blocks)) continue;    // kernel larger than the block
+    float t = 0.f;
+    rc = time_shape(s, b, b, &t);
+    if (rc) return rc;
+    if ((double)t * blocks.size() < best_ms) {
+      best_ms = (double)t * blocks.size();
+      best_bs = b;
+      best_q = b;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_block_cache[key] = {best_bs, best_q};
+  }
+  *bs_out = best_bs;
+  *q_out = best_q;
   return 0;
 }
 
@@ -892,7 +956,8 @@ int build_geometry(mvd_session* s, int v) {
   Int3 bs = s->use_blocks ? s->block
                           : Int3{s->img.x + view.kd.x - 1, s->img.y + view.kd.y - 1, s->img.z + view.kd.z - 1};
   Int3 q;
-  int rc0 = choose_fft_shape(s, bs, &q);
+  int rc0 = !s->use_blocks && s->fft_policy == MVD_FFT_AUTOTUNE ? choose_blocks(s, view.kd, &bs, &q)
+                                                                : choose_fft_shape(s, bs, &q);
   if (rc0) return rc0;
   view.plan = s->plans.get(q);
   if (!view.plan) return -1;

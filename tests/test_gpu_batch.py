@@ -65,12 +65,16 @@ def test_batch_graph_replays_match_eager():
     d2.close()
 
 
-@pytest.mark.parametrize("policy", [FFTPolicy.POW2, FFTPolicy.AUTOTUNE])
-def test_fft_policies_match_oracle(policy):
-    ds, d = _stack((50, 45, 30), 3, 51, policy=int(policy), run=True, iters=4)
+@pytest.mark.parametrize("policy,size", [(FFTPolicy.POW2, (50, 45, 30)), (FFTPolicy.AUTOTUNE, (50, 45, 30)),
+                                         (FFTPolicy.AUTOTUNE, (300, 280, 40))])
+def test_fft_policies_match_oracle(policy, size):
+    """The autotuner may also split an unblocked stack into 256 x 256 x bz
+    blocks (fused plane kernels); any choice must stay within the gate of the
+    whole-image oracle."""
+    ds, d = _stack(size, 3, 51, policy=int(policy), run=True, iters=4)
     psi = d.get_psi()
     geo = d.geometry(0)
-    print("policy", policy.name, "fft", geo["fft"], flush=True)
+    print("policy", policy.name, size, "fft", geo["fft"], "blocks", geo["blocks"], flush=True)
     if policy == FFTPolicy.POW2:
         assert geo["fft"] == (64, 64, 64)
     d.close()
