@@ -370,7 +370,7 @@ def main():
     dc = per_class[dom]
     achieved = dc["gbs"]
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get(dom)

@@ -70,7 +70,7 @@ struct Spec<256> {
 };
 template <>
 struct Spec<512> {
-  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8;
+  static constexpr int E = 16, R0 = 8, R1 = 8, R2 = 8;
 };
 
 // Butterflies of one pass on the registers (inputs already in pass order).
