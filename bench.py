@@ -207,11 +207,11 @@ def run_c4(args):
     # NLL partials and gradients returned to the host)
     kernels = [v.get_kernel1() for v in d.get_data().get_views()]
     for _ in range(args.warmup):
-        d.nll_grad(psfs=kernels)
+        d.nll_grad(psfs=kernels, psi_on_device=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        nll, gpsi, gpsf = d.nll_grad(psfs=kernels)
+        nll, _, gpsf = d.nll_grad(psfs=kernels, psi_on_device=True)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     value = n * nv * args.steps / sec / 1e9

@@ -162,9 +162,20 @@ int mvd_set_psf(mvd_session* s, int view, const float* psf);
  * with the adjoint folding the mirror-single extension back.  psi == NULL uses
  * the session estimate; psfs == NULL (or psfs[v] == NULL) uses the views' PSFs
  * as given (not normalised).  grad_psi (n voxels) and grad_psf[v] (the view's
- * kernel box) are optional.  L and the psi gradient accumulate in FP64. */
+ * kernel box) are optional.  L and the psi gradient accumulate in FP64.
+ * Once the session is prepared the gradient runs over the deconvolution's own
+ * blocks (forward model and residual on every block first, then per block the
+ * window spectrum is recomputed rather than kept: checkpointing, one block's
+ * spectra in memory at a time); before that, over the single window
+ * img + K - 1 (FFT lengths up to 1024). */
 int mvd_nll_grad(mvd_session* s, const float* psi, const float* const* psfs, double* nll, float* grad_psi,
                  float* const* grad_psf);
+/* As mvd_nll_grad, but dL/dpsi (when want_grad_psi) stays in device memory
+ * until mvd_get_grad_psi copies it out (no counterpart in the reference: for
+ * callers that keep the estimate on the GPU). */
+int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* psfs, double* nll, int want_grad_psi,
+                        float* const* grad_psf);
+int mvd_get_grad_psi(mvd_session* s, float* out);
 
 /* Per-kernel-class GPU timing (HIP events on the session stream).
  * mvd_set_timing(s, 1) enables and resets; mvd_get_timing fills up to cap

@@ -21,7 +21,8 @@ EXPORTS = (
     "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
     "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
-    "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_set_fft_policy",
+    "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_nll_grad_device",
+    "mvd_get_grad_psi", "mvd_set_fft_policy",
     "mvd_batch_create", "mvd_batch_add", "mvd_batch_prepare", "mvd_batch_iterate", "mvd_batch_destroy",
 )
 
@@ -63,6 +64,8 @@ def _declare(L):
     L.mvd_get_geometry.argtypes = [_P, ctypes.c_int, _P]
     L.mvd_last_error.restype = ctypes.c_char_p
     L.mvd_nll_grad.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.mvd_nll_grad_device.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P]
+    L.mvd_get_grad_psi.argtypes = [_P, _P]
     L.mvd_set_fft_policy.argtypes = [_P, ctypes.c_int]
     L.mvd_batch_create.argtypes = [ctypes.c_int, _P]
     L.mvd_batch_add.argtypes = [_P, _P]

@@ -3,6 +3,7 @@
 // from a correlation box, and the fold of the adjoint convolution over the
 // mirror-single extension back onto the image.
 #include "mvd_internal.h"
+#include "pointwise_math.h"
 
 namespace mvd {
 
@@ -28,10 +29,11 @@ hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, in
   return hipGetLastError();
 }
 
-// grad[j] = scale * box[(c - j) mod q]: the correlation corr(R, W)(s) evaluated
-// at s = c - j for every kernel voxel j.
+// grad[j] (+)= scale * box[(c - j) mod q]: the correlation corr(R, W)(s)
+// evaluated at s = c - j for every kernel voxel j; accumulate adds the block's
+// share to the view's gradient.
 __global__ __launch_bounds__(kGThreads) void k_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale,
-                                                               float* grad) {
+                                                               float* grad, int accumulate) {
   const int64_t n = (int64_t)kd.x * kd.y * kd.z;
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads) {
     const int x = (int)(i % kd.x), y = (int)((i / kd.x) % kd.y), z = (int)(i / ((int64_t)kd.x * kd.y));
@@ -39,56 +41,59 @@ __global__ __launch_bounds__(kGThreads) void k_extract_psf_grad(const float* box
     if (sx < 0) sx += q.x;
     if (sy < 0) sy += q.y;
     if (sz < 0) sz += q.z;
-    grad[i] = box[sx + (int64_t)q.x * (sy + (int64_t)q.y * sz)] * scale;
+    const float v = box[sx + (int64_t)q.x * (sy + (int64_t)q.y * sz)] * scale;
+    grad[i] = accumulate ? grad[i] + v : v;
   }
 }
 
-hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, hipStream_t stream) {
+hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, int accumulate,
+                                   hipStream_t stream) {
   const int64_t n = (int64_t)kd.x * kd.y * kd.z;
   const int64_t blocks = std::min<int64_t>((n + kGThreads - 1) / kGThreads, 4096);
   (void)hipGetLastError();
-  hipLaunchKernelGGL(k_extract_psf_grad, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, box, q, kd, scale, grad);
+  hipLaunchKernelGGL(k_extract_psf_grad, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, box, q, kd, scale, grad,
+                     accumulate);
   return hipGetLastError();
 }
 
-// Window positions along one axis whose mirror-single image is p: with period
-// P = 2n - 2 these are u = p + kP and u = -p + kP inside the window [-c, n + c)
-// (the two families coincide for p = 0 and p = n - 1).  Requires n >= 2 and
-// c <= 2 (n - 1), which the host checks.  Returns the window indices u + c.
-constexpr int kMaxPre = 16;
-__device__ __forceinline__ int preimages(int p, int n, int c, int* w) {
-  const int P = 2 * n - 2;
-  int k = 0;
-  for (int kk = -3; kk <= 3; ++kk) {
-    const int u1 = p + kk * P;
-    if (u1 >= -c && u1 < n + c && k < kMaxPre) w[k++] = u1 + c;
-    if (p != 0 && p != n - 1) {
-      const int u2 = -p + kk * P;
-      if (u2 >= -c && u2 < n + c && k < kMaxPre) w[k++] = u2 + c;
-    }
-  }
-  return k;
-}
-
-__global__ __launch_bounds__(kGThreads) void k_fold_adjoint(const float* win, Int3 wd, Int3 n, Int3 c, double* acc) {
-  const int64_t total = (int64_t)n.x * n.y * n.z;
+// dst (dense, dims size) = src[origin + (x, y, z)] of an image with dims n: the
+// residual of one block's effective region.
+__global__ __launch_bounds__(kGThreads) void k_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size) {
+  const int64_t total = (int64_t)size.x * size.y * size.z;
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
-    const int x = (int)(i % n.x), y = (int)((i / n.x) % n.y), z = (int)(i / ((int64_t)n.x * n.y));
-    int ux[kMaxPre], uy[kMaxPre], uz[kMaxPre];
-    const int nx = preimages(x, n.x, c.x, ux), ny = preimages(y, n.y, c.y, uy), nz = preimages(z, n.z, c.z, uz);
-    double s = 0.0;
-    for (int a = 0; a < nz; ++a)
-      for (int b = 0; b < ny; ++b)
-        for (int d = 0; d < nx; ++d) s += (double)win[ux[d] + (int64_t)wd.x * (uy[b] + (int64_t)wd.y * uz[a])];
-    acc[i] += s;
+    const int x = (int)(i % size.x), y = (int)((i / size.x) % size.y), z = (int)(i / ((int64_t)size.x * size.y));
+    dst[i] = src[(int64_t)(origin.x + x) + (int64_t)n.x * ((origin.y + y) + (int64_t)n.y * (origin.z + z))];
   }
 }
 
-hipError_t launch_fold_adjoint(const float* win, Int3 wdim, Int3 n, Int3 c, double* acc, hipStream_t stream) {
-  const int64_t total = (int64_t)n.x * n.y * n.z;
+hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size, hipStream_t stream) {
+  const int64_t total = (int64_t)size.x * size.y * size.z;
   const int64_t blocks = std::min<int64_t>((total + kGThreads - 1) / kGThreads, 16384);
+  if (blocks == 0) return hipSuccess;
   (void)hipGetLastError();
-  hipLaunchKernelGGL(k_fold_adjoint, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wdim, n, c, acc);
+  hipLaunchKernelGGL(k_crop, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, dst, src, n, origin, size);
+  return hipGetLastError();
+}
+
+// Adjoint of the mirror-single window gather: every window position u in
+// [lo, lo + wd) adds its value to the image voxel mirror(u).  Positions of one
+// window can share a voxel near the borders, hence the FP64 atomic adds
+// (accumulation order, and with it the last bits, is not fixed).
+__global__ __launch_bounds__(kGThreads) void k_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc) {
+  const int64_t total = (int64_t)wd.x * wd.y * wd.z;
+  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
+    const int x = (int)(i % wd.x), y = (int)((i / wd.x) % wd.y), z = (int)(i / ((int64_t)wd.x * wd.y));
+    const int px = mirror_single(lo.x + x, n.x), py = mirror_single(lo.y + y, n.y), pz = mirror_single(lo.z + z, n.z);
+    unsafeAtomicAdd(acc + (px + (int64_t)n.x * (py + (int64_t)n.y * pz)), (double)win[i]);
+  }
+}
+
+hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream) {
+  const int64_t total = (int64_t)wd.x * wd.y * wd.z;
+  const int64_t blocks = std::min<int64_t>((total + kGThreads - 1) / kGThreads, 16384);
+  if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_fold_window, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, acc);
   return hipGetLastError();
 }
 

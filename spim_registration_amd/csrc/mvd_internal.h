@@ -215,9 +215,11 @@ hipError_t launch_reduce_stats_batch(const ReduceItem* items, int nitems, hipStr
 hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, int conj_a, int conj_b, int64_t n,
                                hipStream_t stream);
 // grad[j] = scale * box[(c - j) mod q] for the kernel box kd (x-fastest).
-hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, hipStream_t stream);
+hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, int accumulate,
+                                   hipStream_t stream);
+hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size, hipStream_t stream);
+hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream);
 // acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
-hipError_t launch_fold_adjoint(const float* win, Int3 wdim, Int3 n, Int3 c, double* acc, hipStream_t stream);
 hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
 hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream);
 

@@ -419,6 +419,8 @@ struct mvd_session {
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
   DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
+  DeviceBuf g_crop, g_nll, g_gpsi;
+  bool g_gpsi_ready = false;
   PlanCache plans;
   Timer timer;
   // The blocks of a view convolved together: one launch per pass and kernel
@@ -1325,8 +1327,8 @@ int mvd_get_psi(mvd_session* s, float* out, int masked) {
   return 0;
 }
 
-int mvd_nll_grad(mvd_session* s, const float* psi, const float* const* psfs, double* nll, float* grad_psi,
-                 float* const* grad_psf) {
+int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* psfs, double* nll, int want_grad_psi,
+                        float* const* grad_psf) {
   MVD_REQUIRE(s, "NULL session");
   MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
   MVD_REQUIRE(psi || s->psi_ready, "no estimate: pass psi or call mvd_init_psi");
@@ -1339,132 +1341,161 @@ int mvd_nll_grad(mvd_session* s, const float* psi, const float* const* psfs, dou
   }
   MVD_REQUIRE(wmode != MVD_WEIGHTS_VIRTUAL, "virtual weights are not supported by mvd_nll_grad");
   const Int3 n = s->img;
-  MVD_REQUIRE(n.x >= 2 && n.y >= 2 && n.z >= 2, "image must have at least 2 voxels per axis");
   hipStream_t st = s->stream;
+  s->g_gpsi_ready = false;
   const float* psrc = s->d_psi.as<float>();
   if (psi) {
     MVD_CHECK(s->g_psi.ensure(sizeof(float) * s->n));
     MVD_CHECK(hipMemcpyAsync(s->g_psi.p, psi, sizeof(float) * s->n, hipMemcpyHostToDevice, st));
     psrc = s->g_psi.as<float>();
   }
-  if (grad_psi) {
+  if (want_grad_psi) {
     MVD_CHECK(s->g_acc.ensure(sizeof(double) * s->n));
     MVD_CHECK(launch_zero_double(s->g_acc.as<double>(), s->n, st));
   }
+  MVD_CHECK(s->g_nll.ensure(sizeof(double) * 2 * s->nviews));
   float* resid = s->d_ratio.as<float>();
-  double total = 0.0;
   for (int v = 0; v < s->nviews; ++v) {
     auto& view = s->views[v];
     const Int3 k = view.kd;
-    const Int3 c = {k.x / 2, k.y / 2, k.z / 2};
-    MVD_REQUIRE(c.x <= 2 * (n.x - 1) && c.y <= 2 * (n.y - 1) && c.z <= 2 * (n.z - 1),
-                "kernel too large for the image (half size > 2 (n - 1))");
     std::vector<float> pk = view.psf_raw;
     if (psfs && psfs[v]) pk.assign(psfs[v], psfs[v] + pk.size());
-    ConvGeom gw{};
-    gw.src = n;
-    gw.off = {-c.x, -c.y, -c.z};
-    gw.win = {n.x + k.x - 1, n.y + k.y - 1, n.z + k.z - 1};
-    gw.q = fft_shape_for(gw.win);
-    gw.eff_local = c;
-    gw.eff_size = n;
-    gw.eff_offset = {0, 0, 0};
-    gw.dst = n;
-    const ShapePlan* spp = s->plans.get(gw.q);
-    if (!spp) return -1;
+    // the deconvolution's own blocks once prepared, else the single window img + K - 1
+    std::vector<ConvGeom> blocks;
+    const ShapePlan* spp = nullptr;
+    if (s->prepared && view.plan) {
+      blocks = view.blocks;
+      spp = view.plan;
+    } else {
+      const Int3 win = {n.x + k.x - 1, n.y + k.y - 1, n.z + k.z - 1};
+      const Int3 q = fft_shape_for(win);
+      spp = s->plans.get(q);
+      MVD_REQUIRE(spp, "no FFT plan for the gradient window (lengths up to 1024): prepare with blocks first");
+      MVD_REQUIRE(divide_into_blocks(n, k, win, q, &blocks), "internal: gradient window");
+    }
     const ShapePlan& sp = *spp;
+    const Int3 q = sp.q;
     const int64_t hs = sp.spectrum_elems();
+    int64_t crop = 0, box = (int64_t)q.x * q.y * q.z;
+    int nparts = 0;
+    for (auto& g : blocks) {
+      crop = std::max<int64_t>(crop, (int64_t)g.eff_size.x * g.eff_size.y * g.eff_size.z);
+      nparts += conv_inverse_workgroups(sp, g);
+    }
     MVD_CHECK(s->g_spec_a.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_b.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_c.ensure(sizeof(float2) * hs));
-    MVD_CHECK(s->g_box.ensure(sizeof(float) * (size_t)gw.q.x * gw.q.y * gw.q.z));
+    MVD_CHECK(s->g_box.ensure(sizeof(float) * box));
+    MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
+    MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
+    MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nparts));
     int rc = kernel_spectrum(s, sp, pk, k, &s->g_khat);
     if (rc) return rc;
     float2* A = s->g_spec_a.as<float2>();
     float2* B = s->g_spec_b.as<float2>();
     float2* Cs = s->g_spec_c.as<float2>();
     const float2* Kh = s->g_khat.as<float2>();
-    // W^ = FFT(E_mirror(psi) on the window)
-    MVD_CHECK(launch_x_forward(sp, gw, psrc, kGatherMirror, A, st));
-    MVD_CHECK(launch_y(sp, A, 0, sp.q.z, 0, sp.q.y, -1.f, st));
-    MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, sp.q.z, st, -1.f));
-    // blur = IFFT(W^ P^); residual and likelihood partials in the epilogue
-    MVD_CHECK(launch_spec_product(B, A, Kh, 0, 0, hs, st));
-    MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, sp.q.z, st, 1.f));
-    MVD_CHECK(launch_y(sp, B, c.z, c.z + n.z, c.y, c.y + n.y, 1.f, st));
-    const int nwg = x_inverse_workgroups(sp, gw);
-    MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nwg));
-    MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nwg));
-    EpilogueArgs e{};
-    e.mode = kEpiResidual;
-    e.dst = resid;
-    e.img = s->d_views.as<float>() + v * s->n;
-    e.weight_mode = wmode == MVD_WEIGHTS_NONE ? kWeightNone : kWeightArray;
-    e.weight = wmode == MVD_WEIGHTS_NONE ? nullptr : s->d_weights.as<float>() + v * s->n;
-    e.part_sum = s->g_part_sum.as<double>();
-    e.part_max = s->g_part_max.as<float>();
-    MVD_CHECK(launch_x_inverse(sp, gw, B, e, st, nullptr));
-    std::vector<double> parts(nwg);
-    MVD_CHECK(hipMemcpyAsync(parts.data(), s->g_part_sum.p, sizeof(double) * nwg, hipMemcpyDeviceToHost, st));
-    MVD_CHECK(hipStreamSynchronize(st));
-    for (double x : parts) total += x;
-    if (!grad_psi && !(grad_psf && grad_psf[v])) continue;
-    // R^ = FFT(residual, zero outside the image, on the window)
-    ConvGeom gr = gw;
-    MVD_CHECK(launch_x_forward(sp, gr, resid, kGatherZero, Cs, st));
-    MVD_CHECK(launch_y(sp, Cs, 0, sp.q.z, 0, sp.q.y, -1.f, st));
-    MVD_CHECK(launch_z(sp, Cs, nullptr, 1.f, 0, sp.q.z, st, -1.f));
-    if (grad_psf && grad_psf[v]) {
-      // dL/dP(j) = corr(R, W)(c - j) = IFFT(conj(R^) W^)(c - j) / (qx qy qz)
-      MVD_CHECK(launch_spec_product(B, Cs, A, 1, 0, hs, st));
-      MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, sp.q.z, st, 1.f));
-      MVD_CHECK(launch_y(sp, B, 0, sp.q.z, 0, sp.q.y, 1.f, st));
-      ConvGeom gb{};
-      gb.src = sp.q;
-      gb.win = sp.q;
-      gb.q = sp.q;
-      gb.eff_local = {0, 0, 0};
-      gb.eff_size = sp.q;
-      gb.eff_offset = {0, 0, 0};
-      gb.dst = sp.q;
-      EpilogueArgs es{};
-      es.mode = kEpiStore;
-      es.dst = s->g_box.as<float>();
-      MVD_CHECK(launch_x_inverse(sp, gb, B, es, st, nullptr));
-      const int64_t kn = (int64_t)k.x * k.y * k.z;
-      MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kn));
-      const float scale = (float)(1.0L / ((long double)sp.q.x * sp.q.y * sp.q.z));
-      MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), sp.q, k, scale, s->g_kgrad.as<float>(), st));
+    // pass 1, every block: b = E_mirror(psi) * P on the effective region, with the
+    // residual r = w (1 - y / b) and the NLL partials w (b - y ln b) in the epilogue
+    int off = 0;
+    for (auto& g : blocks) {
+      EpilogueArgs e{};
+      e.mode = kEpiResidual;
+      e.dst = resid;
+      e.img = s->d_views.as<float>() + v * s->n;
+      e.weight_mode = wmode == MVD_WEIGHTS_NONE ? kWeightNone : kWeightArray;
+      e.weight = wmode == MVD_WEIGHTS_NONE ? nullptr : s->d_weights.as<float>() + v * s->n;
+      e.part_sum = s->g_part_sum.as<double>() + off;
+      e.part_max = s->g_part_max.as<float>() + off;
+      rc = run_conv(s, sp, g, psrc, kGatherMirror, Kh, e, A);
+      if (rc) return rc;
+      off += conv_inverse_workgroups(sp, g);
+    }
+    MVD_CHECK(launch_reduce_stats(s->g_part_sum.as<double>(), s->g_part_max.as<float>(), off,
+                                  s->g_nll.as<double>() + 2 * v, st));
+    const bool want_psf = grad_psf && grad_psf[v];
+    if (!want_grad_psi && !want_psf) continue;
+    const int64_t kn = (int64_t)k.x * k.y * k.z;
+    if (want_psf) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kn));
+    // pass 2, every block, with the window spectrum recomputed instead of kept
+    // (checkpointing: one block's spectra in memory at a time)
+    for (size_t bi = 0; bi < blocks.size(); ++bi) {
+      const ConvGeom& g = blocks[bi];
+      // W^ = FFT(E_mirror(psi) on the block window)
+      MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, A, st));
+      MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
+      // R^ = FFT(residual of this block's effective region, zero elsewhere in the window)
+      MVD_CHECK(launch_crop(s->g_crop.as<float>(), resid, n, g.eff_offset, g.eff_size, st));
+      ConvGeom gc = g;
+      gc.src = g.eff_size;
+      gc.off = {-g.eff_local.x, -g.eff_local.y, -g.eff_local.z};
+      MVD_CHECK(launch_xy_forward_passes(sp, gc, s->g_crop.as<float>(), kGatherZero, Cs, st));
+      MVD_CHECK(launch_z(sp, Cs, nullptr, 1.f, 0, q.z, st, -1.f));
+      if (want_psf) {
+        // dL/dP(j) += corr(R, W)(c - j) = IFFT(conj(R^) W^)(c - j) / (qx qy qz)
+        MVD_CHECK(launch_spec_product(B, Cs, A, 1, 0, hs, st));
+        MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, q.z, st, 1.f));
+        ConvGeom gb{};
+        gb.src = q;
+        gb.win = q;
+        gb.q = q;
+        gb.eff_size = q;
+        gb.dst = q;
+        EpilogueArgs es{};
+        es.mode = kEpiStore;
+        es.dst = s->g_box.as<float>();
+        MVD_CHECK(launch_yx_inverse_passes(sp, gb, B, es, st, nullptr));
+        const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
+        MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, s->g_kgrad.as<float>(), bi > 0, st));
+      }
+      if (want_grad_psi) {
+        // adjoint on the block window: IFFT(R^ conj(P^)), folded back through the mirror extension
+        MVD_CHECK(launch_spec_product(B, Cs, Kh, 0, 1, hs, st));
+        MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, q.z, st, 1.f));
+        ConvGeom gb{};
+        gb.src = g.win;
+        gb.win = g.win;
+        gb.q = q;
+        gb.eff_size = g.win;
+        gb.dst = g.win;
+        EpilogueArgs es{};
+        es.mode = kEpiStore;
+        es.dst = s->g_box.as<float>();
+        MVD_CHECK(launch_yx_inverse_passes(sp, gb, B, es, st, nullptr));
+        MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
+      }
+    }
+    if (want_psf)
       MVD_CHECK(hipMemcpyAsync(grad_psf[v], s->g_kgrad.p, sizeof(float) * kn, hipMemcpyDeviceToHost, st));
-    }
-    if (grad_psi) {
-      // adjoint on the window: IFFT(R^ conj(P^)), then fold the mirror extension back
-      MVD_CHECK(launch_spec_product(B, Cs, Kh, 0, 1, hs, st));
-      MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, sp.q.z, st, 1.f));
-      MVD_CHECK(launch_y(sp, B, 0, gw.win.z, 0, gw.win.y, 1.f, st));
-      ConvGeom gb{};
-      gb.src = gw.win;
-      gb.win = gw.win;
-      gb.q = sp.q;
-      gb.eff_local = {0, 0, 0};
-      gb.eff_size = gw.win;
-      gb.eff_offset = {0, 0, 0};
-      gb.dst = gw.win;
-      EpilogueArgs es{};
-      es.mode = kEpiStore;
-      es.dst = s->g_box.as<float>();
-      MVD_CHECK(launch_x_inverse(sp, gb, B, es, st, nullptr));
-      MVD_CHECK(launch_fold_adjoint(s->g_box.as<float>(), gw.win, n, c, s->g_acc.as<double>(), st));
-    }
-    MVD_CHECK(hipStreamSynchronize(st));
   }
-  if (grad_psi) {
-    MVD_CHECK(launch_to_float(s->g_acc.as<double>(), resid, s->n, st));
-    MVD_CHECK(hipMemcpyAsync(grad_psi, resid, sizeof(float) * s->n, hipMemcpyDeviceToHost, st));
+  std::vector<double> sums(2 * s->nviews);
+  MVD_CHECK(hipMemcpyAsync(sums.data(), s->g_nll.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st));
+  if (want_grad_psi) {
+    MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
+    MVD_CHECK(launch_to_float(s->g_acc.as<double>(), s->g_gpsi.as<float>(), s->n, st));
   }
   MVD_CHECK(hipStreamSynchronize(st));
+  double total = 0.0;
+  for (int v = 0; v < s->nviews; ++v) total += sums[2 * v];
   if (nll) *nll = total;
+  s->g_gpsi_ready = want_grad_psi != 0;
   return 0;
+}
+
+int mvd_get_grad_psi(mvd_session* s, float* out) {
+  MVD_REQUIRE(s && out, "NULL argument");
+  MVD_REQUIRE(s->g_gpsi_ready, "no estimate gradient: call mvd_nll_grad_device with want_grad_psi first");
+  DeviceGuard guard(s->device);
+  MVD_CHECK(hipMemcpyAsync(out, s->g_gpsi.p, sizeof(float) * s->n, hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int mvd_nll_grad(mvd_session* s, const float* psi, const float* const* psfs, double* nll, float* grad_psi,
+                 float* const* grad_psf) {
+  int rc = mvd_nll_grad_device(s, psi, psfs, nll, grad_psi != nullptr, grad_psf);
+  if (rc || !grad_psi) return rc;
+  return mvd_get_grad_psi(s, grad_psi);
 }
 
 int mvd_get_kernels(mvd_session* s, int view, float* kernel1, float* kernel2) {

@@ -251,10 +251,12 @@ class MVDeconvolution:
         _native.check(_native.lib().mvd_get_geometry(self._s.handle, view, g), "mvd_get_geometry")
         return dict(blocks=g[0], fft=(g[1], g[2], g[3]))
 
-    def nll_grad(self, psi=None, psfs=None, want_psi: bool = True, want_psf: bool = True):
+    def nll_grad(self, psi=None, psfs=None, want_psi: bool = True, want_psf: bool = True,
+                 psi_on_device: bool = False):
         """Poisson NLL of the forward model (b_v = E_mirror(psi) * P_v) and its
         gradient w.r.t. the estimate and each view's PSF (mvd_nll_grad).
         psi / psfs default to the current estimate and the views' PSFs.
+        psi_on_device keeps d/dpsi in device memory (fetch it with grad_psi()).
         Returns (nll, grad_psi or None, [grad_psf_v] or None)."""
         L = _native.lib()
         s = self._s
@@ -265,14 +267,24 @@ class MVDeconvolution:
             ks = [_f32(k) for k in psfs]
             keep += ks
             kp = (ctypes.c_void_p * len(ks))(*[k.ctypes.data for k in ks])
-        gpsi = np.empty(s.shape, np.float32) if want_psi else None
         gpsf = [np.empty_like(v.kernel1) for v in s.views] if want_psf else None
         gp = None
         if gpsf is not None:
             gp = (ctypes.c_void_p * len(gpsf))(*[g.ctypes.data for g in gpsf])
         nll = ctypes.c_double()
+        if psi_on_device:
+            _native.check(L.mvd_nll_grad_device(s.handle, _ptr(p0), kp, ctypes.byref(nll), 1 if want_psi else 0, gp),
+                          "mvd_nll_grad_device")
+            return nll.value, None, gpsf
+        gpsi = np.empty(s.shape, np.float32) if want_psi else None
         _native.check(L.mvd_nll_grad(s.handle, _ptr(p0), kp, ctypes.byref(nll), _ptr(gpsi), gp), "mvd_nll_grad")
         return nll.value, gpsi, gpsf
+
+    def grad_psi(self) -> np.ndarray:
+        """d/dpsi of the last nll_grad(..., psi_on_device=True) (mvd_get_grad_psi)."""
+        out = np.empty(self._s.shape, np.float32)
+        _native.check(_native.lib().mvd_get_grad_psi(self._s.handle, _ptr(out)), "mvd_get_grad_psi")
+        return out
 
     def set_timing(self, on: bool):
         _native.check(_native.lib().mvd_set_timing(self._s.handle, 1 if on else 0), "mvd_set_timing")
