@@ -199,7 +199,9 @@ def run_c4(args):
     inp = MVDeconInput()
     for v, p in zip(ds.views, start):
         inp.add(MVDeconFFT(v, None, p, [0]))
-    d = MVDeconvolution(inp, PSFTYPE.EFFICIENT_BAYESIAN, 0, 0.0, run=False)
+    from spim_registration_amd import FFTPolicy
+    d = MVDeconvolution(inp, PSFTYPE.EFFICIENT_BAYESIAN, 0, 0.0, run=False, fft_policy=int(FFTPolicy.AUTOTUNE))
+    log(f"geometry {d.geometry(0)} (autotuned blocks and FFT shape)")
     d.iterate(2)
     nv = w["nviews"]
     n = int(np.prod(w["size_xyz"]))
@@ -291,6 +293,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of 256 MiB (counter calibration for traffic)")
     ap.add_argument("--refine-steps", type=int, default=50, help="c4: PSF refinement steps")
+    ap.add_argument("--grad", type=int, default=0,
+                    help="c2/c5: also time this many likelihood-gradient evaluations (blocked, checkpointed)")
     ap.add_argument("--batched", action="store_true",
                     help="run the blocks of each view through the batch engine (one launch per pass, hipGraph)")
     args = ap.parse_args()
@@ -380,6 +384,21 @@ def main():
     path_gbs = pb / sec / 1e9
     log("per-kernel: " + json.dumps(per_class, indent=None))
 
+    grad = None
+    if args.grad > 0:
+        kernels = [v.get_kernel1() for v in d.get_data().get_views()]
+        d.nll_grad(psfs=kernels, psi_on_device=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.grad):
+            d.nll_grad(psfs=kernels, psi_on_device=True)
+        torch.cuda.synchronize()
+        gs = (time.perf_counter() - t0) / args.grad
+        grad = {"seconds_per_evaluation": round(gs, 4), "gvoxel_view_grad_eval_per_s": round(n * nv / gs / 1e9, 4),
+                "what": "NLL, d/dpsi and d/dPSF of every view over the deconvolution's blocks, window spectra "
+                        "recomputed per block (checkpointing)"}
+        log(f"gradient: {gs * 1e3:.1f} ms per evaluation")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -417,6 +436,8 @@ def main():
                     for k, v in per_class.items()},
         "cpu_baseline": cpu,
     }
+    if grad:
+        out["gradient"] = grad
     print(json.dumps(out), flush=True)
 
 
