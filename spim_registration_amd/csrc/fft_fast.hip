@@ -102,18 +102,24 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
       if (valid) v[b * S::R0 + t] = gmem(P.spec)[z * plane + off];
     }
   auto addr = [line](int i) { return i * LP + line; };
-  fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
   constexpr int RL = last_radix<N>();
+  // the kernel spectrum is loaded with the line, in the order the forward
+  // transform leaves its outputs, so its latency hides behind the transform
+  float2 kv[E];
   if (P.kspec) {
 #pragma unroll
     for (int b = 0; b < E / RL; ++b)
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int z = elem_index<N, E, RL>(jt, b, t);
-        float2 k = make_float2(0.f, 0.f);
-        if (valid) k = gmem(P.kspec)[z * plane + off];
-        v[b * RL + t] = cmul(v[b * RL + t], k);
+        kv[b * RL + t] = make_float2(0.f, 0.f);
+        if (valid) kv[b * RL + t] = gmem(P.kspec)[z * plane + off];
       }
+  }
+  fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
+  if (P.kspec) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
     fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
     if (valid) {
 #pragma unroll
