@@ -214,13 +214,11 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
       const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
       float2* row = lds + l * kXYMP;
       const float2 a = row[k], b = row[(kXYM - k) & (kXYM - 1)];
-      const float2 bc = cconj(b), ac = cconj(a);
-      const float2 E1 = cscale(cadd(a, bc), 0.5f), D1 = cscale(csub(a, bc), 0.5f);
-      const float2 E2 = cscale(cadd(b, ac), 0.5f), D2 = cscale(csub(b, ac), 0.5f);
-      const float2 xk = cadd(E1, cmul(tw[k], make_float2(D1.y, -D1.x)));
-      const float2 xm = cadd(E2, cmul(tw[kXYM - k], make_float2(D2.y, -D2.x)));
-      row[k] = xk;
-      if (kXYM - k != k) row[kXYM - k] = xm;
+      const float2 E = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+      const float2 O = make_float2(0.5f * (a.y + b.y), 0.5f * (b.x - a.x));
+      const float2 T = cmul(tw[k], O);
+      row[k] = cadd(E, T);
+      if (kXYM - k != k) row[kXYM - k] = make_float2(E.x - T.x, T.y - E.y);
     }
     __syncthreads();
     // this half's rows of my columns; column 0 carries DC + i Nyquist
@@ -279,6 +277,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int tid = threadIdx.x;
   const ConvGeom& g = P.g;
   const EpilogueArgs& e = P.e;
+  const double inv_lambda = rl_inv_lambda(e.lambda);
   const int zo = bx;                 // output plane inside the effective box
   const int z = g.eff_local.z + zo;  // plane of the FFT box
   if (tid < kXYN) tw[tid] = gmem(P.tw)[tid];
@@ -407,7 +406,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
           w = (float)fmin(1.0, r * e.osem);
         }
         const float last = in0[i];
-        const float next = rl_next_value(last, val, w, e.lambda);
+        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
         gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
@@ -430,12 +429,11 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
       float2* row = lds + l * kXYMP;
       const float2 a = row[k], b = row[kXYM - k];
-      const float2 za = cadd(cadd(a, cconj(b)), mul_di(cmul(csub(a, cconj(b)), cconj(tw[k])), 1.f));
-      if (k != 0 && kXYM - k != k) {
-        const float2 zb = cadd(cadd(b, cconj(a)), mul_di(cmul(csub(b, cconj(a)), cconj(tw[kXYM - k])), 1.f));
-        row[kXYM - k] = zb;
-      }
-      row[k] = za;
+      const float2 A = make_float2(a.x + b.x, a.y - b.y);
+      const float2 B = make_float2(a.x - b.x, a.y + b.y);
+      const float2 C = cmul(B, cconj(tw[k]));
+      if (k != 0 && kXYM - k != k) row[kXYM - k] = make_float2(A.x + C.y, C.x - A.y);
+      row[k] = make_float2(A.x - C.y, A.y + C.x);
     }
     __syncthreads();
     xy_rows_fft(lds, tw, 1.f);

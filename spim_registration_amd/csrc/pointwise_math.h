@@ -20,14 +20,18 @@ __device__ __forceinline__ int mirror_single(int p, int n) {
 }
 
 // MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726).
-__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda) {
+// inv_lambda = 1 / lambda, computed once by the caller: the division by lambda
+// becomes a multiplication (the double result may move by one ulp; the float
+// it is rounded to practically never does).
+__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda,
+                                               double inv_lambda) {
 #pragma clang fp contract(off)
   const float minv = 0.0001f;
   const float value = last * integral;
   float adjusted;
   if (value > 0.f) {
     if (lambda > 0.0)
-      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
+      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) * inv_lambda);
     else
       adjusted = value;
   } else {
@@ -36,6 +40,8 @@ __device__ __forceinline__ float rl_next_value(float last, float integral, float
   const float next = isnan(adjusted) ? minv : fmaxf(minv, adjusted);
   return last + ((next - last) * weight);
 }
+
+__device__ __forceinline__ double rl_inv_lambda(double lambda) { return lambda > 0.0 ? 1.0 / lambda : 0.0; }
 
 // Quotient of the observed view by the blurred estimate (MVDeconvolution.computeQuotient, deconv/MVDeconvolution.java:494-546).
 __device__ __forceinline__ float rl_quotient(float blurred, float img) {
