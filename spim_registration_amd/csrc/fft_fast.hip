@@ -38,7 +38,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 #pragma unroll
     for (int t = 0; t < S::R0; ++t) {
       const int y = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = valid ? base[(int64_t)y * P.pitch] : make_float2(0.f, 0.f);
+      v[b * S::R0 + t] = valid ? base[y * P.pitch] : make_float2(0.f, 0.f);
     }
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
@@ -49,7 +49,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int y = elem_index<N, E, RL>(jt, b, t);
-        if (y >= P.ylo && y < P.yhi) base[(int64_t)y * P.pitch] = v[b * RL + t];
+        if (y >= P.ylo && y < P.yhi) base[y * P.pitch] = v[b * RL + t];
       }
   }
 }
@@ -90,8 +90,9 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
   const int col = bx * L + line;
   const bool valid = col < P.hx;
-  const int64_t plane = (int64_t)P.qy * P.pitch;
-  const int64_t off = (int64_t)by * P.pitch + col;
+  // spectra hold fewer than 2^31 elements (lengths up to 1024): 32-bit offsets
+  const int plane = P.qy * P.pitch;
+  const int off = by * P.pitch + col;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
@@ -391,20 +392,19 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   const int lsub = RPI > 1 ? threadIdx.x / W : 0;
   constexpr int NR = L / RPI;
   // phase 1: destination indices and the inputs of the epilogue, all rows
-  int64_t gidx[NR][EPT];
+  int gidx[NR][EPT];  // images hold fewer than 2^31 voxels (checked by the host)
   float in0[NR][EPT], in1[NR][EPT];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const int row = row0 + r * RPI + lsub;
     const bool live_row = row < nrows;
     const int yo = live_row ? row % ny : 0, zo = live_row ? row / ny : 0;
-    const int64_t base = (int64_t)g.eff_offset.x +
-                         (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + yo) + (int64_t)g.dst.y * (g.eff_offset.z + zo));
+    const int base = g.eff_offset.x + g.dst.x * ((g.eff_offset.y + yo) + g.dst.y * (g.eff_offset.z + zo));
 #pragma unroll
     for (int ee = 0; ee < EPT; ++ee) {
       const int xo = x0 + ee * kThreads;
       const bool live = live_row && xo < nx;
-      const int64_t gi = live ? base + xo : -1;
+      const int gi = live ? base + xo : -1;
       gidx[r][ee] = gi;
       in0[r][ee] = 0.f;
       in1[r][ee] = 1.f;
@@ -425,7 +425,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     const int l = r * RPI + lsub;
 #pragma unroll
     for (int ee = 0; ee < EPT; ++ee) {
-      const int64_t gi = gidx[r][ee];
+      const int gi = gidx[r][ee];
       if (gi < 0) continue;
       const int xo = x0 + ee * kThreads;
       const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];

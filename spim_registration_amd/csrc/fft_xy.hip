@@ -187,9 +187,9 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
         const bool y_in = gy >= 0 && gy < g.src.y;
         if (MODE == kGatherMirror) {
           if (!y_in) gy = mirror_single(gy, g.src.y);
-          val = gmem(P.src)[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+          val = gmem(P.src)[gx + g.src.x * (gy + g.src.y * gz)];
         } else if (z_in && y_in && x_in) {
-          val = gmem(P.src)[gx + (int64_t)g.src.x * (gy + (int64_t)g.src.y * gz)];
+          val = gmem(P.src)[gx + g.src.x * (gy + g.src.y * gz)];
         } else {
           val = MODE == kGatherOne ? 1.f : 0.f;
         }
@@ -239,13 +239,13 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     if (cl == 0)
       dcn[ky] = c0[t];
     else
-      out[(int64_t)ky * P.pitch + cl] = c0[t];
+      out[ky * P.pitch + cl] = c0[t];
   }
   xy_fft<kXYN>(c1, lds, jy, tw, 1, -1.f, yaddr);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    out[(int64_t)ky * P.pitch + 64 + cl] = c1[t];
+    out[ky * P.pitch + 64 + cl] = c1[t];
   }
   // separate the DC and Nyquist columns: Z = F0 + i F128 with F0 and F128 the
   // transforms of real columns (the dcn writes are ordered by the barriers of
@@ -254,8 +254,8 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     const int k = tid;
     const float2 a = dcn[k], b = cconj(dcn[(kXYN - k) & (kXYN - 1)]);
     const float2 s = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
-    out[(int64_t)k * P.pitch] = s;
-    out[(int64_t)k * P.pitch + kXYM] = make_float2(d.y, -d.x);
+    out[k * P.pitch] = s;
+    out[k * P.pitch + kXYM] = make_float2(d.y, -d.x);
   }
 }
 
@@ -287,11 +287,11 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   float2 c0[16], c1[16];
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
-    const float2* row = in + (int64_t)elem_index<kXYN, 16, SY::R0>(jy, 0, t) * P.pitch;
+    const auto row = in + elem_index<kXYN, 16, SY::R0>(jy, 0, t) * P.pitch;
     c0[t] = row[cl];
     c1[t] = row[64 + cl];
   }
-  if (tid < kXYN) nyq[tid] = in[(int64_t)tid * P.pitch + kXYM];
+  if (tid < kXYN) nyq[tid] = in[tid * P.pitch + kXYM];
   __syncthreads();
   if (cl == 0) {
 #pragma unroll
@@ -350,9 +350,9 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int nx = g.eff_size.x;
   const float* fbuf = reinterpret_cast<const float*>(lds);
   constexpr int R = kXYEpiRows;
-  auto out_index = [&](int x, int y, int zo_) -> int64_t {
-    return (int64_t)(g.eff_offset.x + x) +
-           (int64_t)g.dst.x * ((int64_t)(g.eff_offset.y + y - ylo) + (int64_t)g.dst.y * (g.eff_offset.z + zo_));
+  // images hold fewer than 2^31 voxels (checked by the host): 32-bit offsets
+  auto out_index = [&](int x, int y, int zo_) -> int {
+    return (g.eff_offset.x + x) + g.dst.x * ((g.eff_offset.y + y - ylo) + g.dst.y * (g.eff_offset.z + zo_));
   };
   // inputs of the epilogue for rows rsub + 4 (c R + i) of half h; lanes run along x
   auto load_batch = [&](int h, int c, float (&in0)[R], float (&in1)[R]) {
@@ -365,7 +365,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       in0[i] = 0.f;
       in1[i] = 1.f;
       if (x_live && y >= ylo && y < yhi) {
-        const int64_t gi = out_index(x, y, zo);
+        const int gi = out_index(x, y, zo);
         if (EPI == kEpiQuotient || EPI == kEpiResidual) {
           in0[i] = gmem(e.img)[gi];
           if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = gmem(e.weight)[gi];
@@ -385,7 +385,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       const int yl = rsub + 4 * (c * R + i);
       const int y = h * kXYHalf + yl;
       if (!x_live || y < ylo || y >= yhi) continue;
-      const int64_t gi = out_index(x, y, zo);
+      const int gi = out_index(x, y, zo);
       const float val = fbuf[yl * 2 * kXYMP + g.eff_local.x + x];
       if (EPI == kEpiStore) {
         gmem(e.dst)[gi] = val;

@@ -1035,6 +1035,7 @@ int mvd_create(int device, int nx, int ny, int nz, int nviews, mvd_session** out
   s->device = device;
   s->img = {nx, ny, nz};
   s->n = (int64_t)nx * ny * nz;
+  MVD_REQUIRE(s->n < (int64_t(1) << 31), "images of 2^31 voxels or more are not supported");
   s->nviews = nviews;
   s->views.reserve(nviews);
   MVD_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
