@@ -142,6 +142,80 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
   }
 }
 
+// Many blocks convolved with one kernel spectrum (the blocks of one view): a
+// workgroup owns one set of lines (bx, by) and runs it for every item, so the
+// kernel spectrum of its lines is read once instead of once per block.  All
+// items share hx, qy, pitch, the twiddles and kspec (the host checks).
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_z_fast_lines(const ZFastParams* items, int nitems) {
+  using S = Spec<N>;
+  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr int RL = last_radix<N>();
+  __shared__ float2 tw[N];
+  __shared__ float2 lds[N * LP];
+  const ZFastParams P0 = items[0];
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P0.tw)[i];
+  const int col = blockIdx.x * L + line;
+  const bool valid = col < P0.hx;
+  const int plane = P0.qy * P0.pitch;  // spectra hold fewer than 2^31 elements
+  // lanes past the last column load a valid column and discard it: branch-free loads
+  const int off = blockIdx.y * P0.pitch + (valid ? col : P0.hx - 1);
+  auto addr = [line](int i) { return i * LP + line; };
+  float2 kv[E];
+#pragma unroll
+  for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+    for (int t = 0; t < RL; ++t) {
+      const int z = elem_index<N, E, RL>(jt, b, t);
+      kv[b * RL + t] = gmem(P0.kspec)[z * plane + off];
+    }
+#pragma unroll 1
+  for (int it = 0; it < nitems; ++it) {
+    // twiddles re-read from LDS in every item: hoisting them out of the loop
+    // would hold ~60 more registers and cost occupancy
+    const float2* twl = tw + opaque(0);
+    const auto spec = gmem(items[it].spec);
+    const int zlo = items[it].zlo, zhi = items[it].zhi;
+    float2 v[E];
+#pragma unroll
+    for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+      for (int t = 0; t < S::R0; ++t) {
+        const int z = elem_index<N, E, S::R0>(jt, b, t);
+        v[b * S::R0 + t] = spec[z * plane + off];
+      }
+    fft_forward_order<N>(v, lds, jt, twl, 1, -1.f, addr);
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
+    fft_reverse_order<N>(v, lds, jt, twl, 1, 1.f, addr);
+    if (valid) {
+#pragma unroll
+      for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+        for (int t = 0; t < S::R0; ++t) {
+          const int z = elem_index<N, E, S::R0>(jt, b, t);
+          if (z >= zlo && z < zhi) spec[z * plane + off] = v[b * S::R0 + t];
+        }
+    }
+  }
+}
+
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream) {
+  if (nitems <= 0 || gx <= 0 || gy <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  const ZFastParams* it = static_cast<const ZFastParams*>(items);
+  const dim3 grid(gx, gy), block(kThreads);
+  switch (n) {
+    case 64: hipLaunchKernelGGL(k_z_fast_lines<64>, grid, block, 0, stream, it, nitems); break;
+    case 128: hipLaunchKernelGGL(k_z_fast_lines<128>, grid, block, 0, stream, it, nitems); break;
+    case 256: hipLaunchKernelGGL(k_z_fast_lines<256>, grid, block, 0, stream, it, nitems); break;
+    case 512: hipLaunchKernelGGL(k_z_fast_lines<512>, grid, block, 0, stream, it, nitems); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int N>
 __global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
   z_fast_body<N>(P, blockIdx.x, blockIdx.y);

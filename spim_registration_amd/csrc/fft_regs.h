@@ -21,6 +21,14 @@
 
 namespace mvd {
 
+// Returns v unchanged but opaque to the optimiser: values derived from it are
+// recomputed where they are used instead of being hoisted and kept live across
+// phases or loop iterations (register pressure).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <>
 __device__ __forceinline__ void butterfly<16>(float2* v, float dir) {
   // n = 4 n2 + n1, k = k1 + 4 k2

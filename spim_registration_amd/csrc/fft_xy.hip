@@ -44,14 +44,6 @@ static_assert(SX::E == 16 && SY::E == 16 && SX::R2 == 1 && SY::R2 == 1, "fused x
 static_assert(kXYThreads / (kXYM / SX::E) == kXYHalf, "one x row per 8 threads");
 static_assert(kXYThreads / (kXYN / SY::E) == 64, "64 y columns per round");
 
-// Returns v unchanged but opaque to the optimiser, so that index arithmetic
-// derived from it is recomputed in each phase instead of being kept live (and
-// spilled) across the whole kernel: the plane already fills the register file.
-__device__ __forceinline__ int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
 // Full transform of 16 values per thread; outputs in the order of the last
 // radix.  Barriers before the LDS exchange and after it.
 template <int N, class Addr>

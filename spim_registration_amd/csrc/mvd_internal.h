@@ -198,6 +198,8 @@ hipError_t launch_batch_generic(int kind, int epi, const void* items, const int*
                                 int nwg, size_t lds, hipStream_t stream);
 hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
                              int nwg, hipStream_t stream);
+// z passes of many blocks sharing one kernel spectrum and line grid (gx, gy).
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream);
 // Fused x-y items: kind kPassXForward (cls = gather) or kPassXInverse (cls = epilogue).
 int build_xy_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls);
 hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,

@@ -387,6 +387,8 @@ struct PassLaunch {
   int kind = 0, cls = 0, epi = 0, nitems = 0, nwg = 0;
   int path = 0;    // 0 generic kernels, 1 register-resident kernels, 2 fused x-y kernels
   int timing = 0;  // TimingClass of the launch
+  bool lines = false;  // z pass with one kernel spectrum for all items: grid (lx, ly) of lines
+  int lx = 0, ly = 0;
   bool reduce = false;
   size_t lds = 0;
   DeviceBuf items, prefix, gx;
@@ -465,7 +467,13 @@ int timing_class(int kind, int path, int epi) {
 // Groups the items of one pass by kernel class -- (path, template selector,
 // epilogue) -- and appends one launch per group.
 int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaunch>* out) {
-  std::map<std::tuple<int, int, int>, std::vector<const PassDesc*>> groups;
+  // z items (register kernels) that share a kernel spectrum and shape with at
+  // least one other item form their own group: one "lines" launch reads the
+  // kernel spectrum once for all of them (the blocks of one view of one stack)
+  std::map<std::pair<const float2*, const ShapePlan*>, int> kshare;
+  for (auto& d : descs)
+    if (d.kind == kPassZConv && d.kspec && pass_is_fast(d)) ++kshare[{d.kspec, d.sp}];
+  std::map<std::tuple<int, int, int, uintptr_t>, std::vector<const PassDesc*>> groups;
   for (auto& d : descs) {
     int cls = -1, path = 0;
     unsigned char tmp[kMaxParamBytes];
@@ -477,7 +485,10 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       path = 1;
       build_fast_params(d, tmp, &gx, &gy, &cls);
     }
-    groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0}].push_back(&d);
+    uintptr_t lines = 0;
+    if (path == 1 && d.kind == kPassZConv && d.kspec && kshare[{d.kspec, d.sp}] > 1)
+      lines = reinterpret_cast<uintptr_t>(d.kspec);
+    groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0, lines}].push_back(&d);
   }
   for (auto& kv : groups) {
     PassLaunch L;
@@ -503,6 +514,12 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
     }
     L.nitems = (int)kv.second.size();
     L.nwg = prefix.back();
+    if (std::get<3>(kv.first)) {
+      unsigned char tmp[kMaxParamBytes];
+      int cls = 0;
+      build_fast_params(*kv.second.front(), tmp, &L.lx, &L.ly, &cls);
+      L.lines = true;
+    }
     MVD_CHECK(L.items.ensure(blob.size()));
     MVD_CHECK(hipMemcpy(L.items.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     MVD_CHECK(L.prefix.ensure(sizeof(int) * prefix.size()));
@@ -516,6 +533,7 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
 
 hipError_t enqueue_pass_launch(const PassLaunch& L, hipStream_t st) {
   if (L.reduce) return launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, st);
+  if (L.lines) return launch_z_lines(L.cls, L.items.p, L.nitems, L.lx, L.ly, st);
   if (L.path == 2)
     return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
   if (L.path == 1)
