@@ -48,11 +48,15 @@ def kernel_bytes(geom_blocks, q, img_n):
     P = qx * qy * qz
     b = dict(x_forward=0.0, y_forward=0.0, z_conv=0.0, y_inverse=0.0, x_inverse_quotient=0.0,
              x_inverse_update=0.0, xy_forward=0.0, xy_inverse_quotient=0.0, xy_inverse_update=0.0)
+    # with several blocks the z pass of a view reads the kernel spectrum once
+    # for all blocks (one line set per workgroup, looping over the blocks)
+    if len(geom_blocks) > 1:
+        b["z_conv"] += 2 * 8 * H
     for (ex, ey, ez) in geom_blocks:
         E = ex * ey * ez
         b["x_forward"] += 2 * (4 * P + 8 * H)
         b["y_forward"] += 2 * 16 * H
-        b["z_conv"] += 2 * (8 * H + 8 * H + 8 * H * ez / qz)
+        b["z_conv"] += 2 * (8 * H + 8 * H * ez / qz) + (2 * 8 * H if len(geom_blocks) == 1 else 0)
         b["y_inverse"] += 2 * (8 * H * ez / qz + 8 * H * ez * ey / (qz * qy))
         rows = hx * ey * ez * 8
         b["x_inverse_quotient"] += rows + 8 * E
