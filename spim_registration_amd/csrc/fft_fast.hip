@@ -144,10 +144,16 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 
 // Many blocks convolved with one kernel spectrum (the blocks of one view): a
 // workgroup owns one set of lines (bx, by) and runs it for every item, so the
-// kernel spectrum of its lines is read once instead of once per block.  All
-// items share hx, qy, pitch, the twiddles and kspec (the host checks).
+// kernel spectrum of its lines is read once instead of once per block.  The
+// lines of the next item are loaded while the current one is transformed (the
+// items' spectra are distinct buffers).  All items share hx, qy, pitch, the
+// twiddles and kspec, and every spectrum spans fewer than 2^31 bytes (the host
+// checks): memory is accessed through buffer resources with 32-bit offsets,
+// and stores outside [zlo, zhi) are dropped by the range check rather than
+// branched around, so the wait for the prefetch does not depend on how many
+// stores a lane made.
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_z_fast_lines(const ZFastParams* items, int nitems) {
+__global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
   constexpr int RL = last_radix<N>();
@@ -158,46 +164,60 @@ __global__ __launch_bounds__(kThreads) void k_z_fast_lines(const ZFastParams* it
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P0.tw)[i];
   const int col = blockIdx.x * L + line;
   const bool valid = col < P0.hx;
-  const int plane = P0.qy * P0.pitch;  // spectra hold fewer than 2^31 elements
-  // lanes past the last column load a valid column and discard it: branch-free loads
-  const int off = blockIdx.y * P0.pitch + (valid ? col : P0.hx - 1);
+  const uint32_t plane = (uint32_t)(P0.qy * P0.pitch) * 8u;  // bytes
+  const uint32_t bytes = plane * (uint32_t)N;
+  // lanes past the last column load a valid column and discard it
+  const uint32_t off = (uint32_t)(blockIdx.y * P0.pitch + (valid ? col : P0.hx - 1)) * 8u;
   auto addr = [line](int i) { return i * LP + line; };
+  auto load_line = [&](const float2* spec, float2(&v)[E]) {
+    const BufRes r = buf_res(spec, bytes);
+#pragma unroll
+    for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+      for (int t = 0; t < S::R0; ++t) {
+        const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
+        v[b * S::R0 + t] = buf_load2(r, z * plane + off);
+      }
+  };
+  float2 nxt[E];
+  load_line(P0.spec, nxt);
   float2 kv[E];
+  {
+    const BufRes r = buf_res(P0.kspec, bytes);
 #pragma unroll
-  for (int b = 0; b < E / RL; ++b)
+    for (int b = 0; b < E / RL; ++b)
 #pragma unroll
-    for (int t = 0; t < RL; ++t) {
-      const int z = elem_index<N, E, RL>(jt, b, t);
-      kv[b * RL + t] = gmem(P0.kspec)[z * plane + off];
-    }
+      for (int t = 0; t < RL; ++t) {
+        const uint32_t z = elem_index<N, E, RL>(jt, b, t);
+        kv[b * RL + t] = buf_load2(r, z * plane + off);
+      }
+  }
+  // wait for the kernel spectrum here, ahead of the loop and its prefetches
+#pragma unroll
+  for (int i = 0; i < E; ++i) asm volatile("" : "+v"(kv[i].x), "+v"(kv[i].y));
 #pragma unroll 1
   for (int it = 0; it < nitems; ++it) {
     // twiddles re-read from LDS in every item: hoisting them out of the loop
     // would hold ~60 more registers and cost occupancy
     const float2* twl = tw + opaque(0);
-    const auto spec = gmem(items[it].spec);
+    const BufRes out = buf_res(items[it].spec, bytes);
     const int zlo = items[it].zlo, zhi = items[it].zhi;
     float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = nxt[i];
+    if (it + 1 < nitems) load_line(items[it + 1].spec, nxt);
+    fft_forward_order<N>(v, lds, jt, twl, 1, -1.f, addr);
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
+    fft_reverse_order<N>(v, lds, jt, twl, 1, 1.f, addr);
 #pragma unroll
     for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const int z = elem_index<N, E, S::R0>(jt, b, t);
-        v[b * S::R0 + t] = spec[z * plane + off];
+        const bool keep = valid && z >= zlo && z < zhi;
+        buf_store2(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t]);
       }
-    fft_forward_order<N>(v, lds, jt, twl, 1, -1.f, addr);
-#pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
-    fft_reverse_order<N>(v, lds, jt, twl, 1, 1.f, addr);
-    if (valid) {
-#pragma unroll
-      for (int b = 0; b < E / S::R0; ++b)
-#pragma unroll
-        for (int t = 0; t < S::R0; ++t) {
-          const int z = elem_index<N, E, S::R0>(jt, b, t);
-          if (z >= zlo && z < zhi) spec[z * plane + off] = v[b * S::R0 + t];
-        }
-    }
   }
 }
 

@@ -98,6 +98,50 @@ __device__ __forceinline__ auto gmem(T* p) {
 #endif
 }
 
+// Element at a 32-bit byte offset from a wave-uniform base: the compiler can
+// keep the base in SGPRs and use the SGPR-base + 32-bit VGPR-offset global
+// form, one VGPR and one add per address instead of 64-bit address pairs.
+// The buffer behind `base` must span fewer than 2^32 bytes.
+template <class T>
+__device__ __forceinline__ auto gbyte(T* base, uint32_t byte_off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  using C = __attribute__((address_space(1))) char;
+  using R = __attribute__((address_space(1))) T;
+  return (R*)((C*)base + byte_off);
+#else
+  return (T*)((char*)base + byte_off);
+#endif
+}
+
+// Buffer-resource access to a float2 array of `bytes` < 2^31 bytes: the base
+// stays in SGPRs, each lane gives a 32-bit byte offset, and the hardware range
+// check of raw buffers turns an access at offset >= bytes into a dropped store
+// (a load returns zero).  kNoLane (2^31) is such an offset: masked lanes take
+// it instead of a branch around the store, so every lane issues the same
+// memory instructions.  Resource word 3 is the gfx9-family value for raw
+// 32-bit-format buffers.
+constexpr uint32_t kNoLane = 0x80000000u;
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t BufRes;
+__device__ __forceinline__ BufRes buf_res(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t byte_off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void buf_store2(BufRes r, uint32_t byte_off, float2 v) {
+  typedef unsigned int U2 __attribute__((vector_size(8)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, byte_off, 0, 0);
+}
+#else
+struct BufRes {
+  char* p;
+};
+__device__ __forceinline__ BufRes buf_res(const void* base, uint32_t) { return BufRes{(char*)base}; }
+__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t o) { return *(const float2*)(r.p + o); }
+__device__ __forceinline__ void buf_store2(BufRes r, uint32_t o, float2 v) { *(float2*)(r.p + o) = v; }
+#endif
+
 FftPlan make_plan(int n);
 inline int spectrum_pitch(int hx) { return (hx + 15) / 16 * 16; }
 bool fft_size_supported(int n);

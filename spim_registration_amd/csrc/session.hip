@@ -8,6 +8,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -470,9 +471,12 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
   // z items (register kernels) that share a kernel spectrum and shape with at
   // least one other item form their own group: one "lines" launch reads the
   // kernel spectrum once for all of them (the blocks of one view of one stack)
+  // (the lines kernel addresses each spectrum with 32-bit buffer offsets:
+  // spectra of 2^31 bytes or more take the per-item kernel)
   std::map<std::pair<const float2*, const ShapePlan*>, int> kshare;
   for (auto& d : descs)
-    if (d.kind == kPassZConv && d.kspec && pass_is_fast(d)) ++kshare[{d.kspec, d.sp}];
+    if (d.kind == kPassZConv && d.kspec && pass_is_fast(d) && d.sp->spectrum_elems() * 8 < (int64_t(1) << 31))
+      ++kshare[{d.kspec, d.sp}];
   std::map<std::tuple<int, int, int, uintptr_t>, std::vector<const PassDesc*>> groups;
   for (auto& d : descs) {
     int cls = -1, path = 0;
@@ -519,6 +523,11 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       int cls = 0;
       build_fast_params(*kv.second.front(), tmp, &L.lx, &L.ly, &cls);
       L.lines = true;
+      // the lines kernel loads an item's lines while it stores the previous
+      // item's: the spectra must be distinct buffers
+      std::set<const float2*> specs;
+      for (const PassDesc* d : kv.second)
+        MVD_REQUIRE(specs.insert(d->spec).second, "internal: z items of one launch share a spectrum");
     }
     MVD_CHECK(L.items.ensure(blob.size()));
     MVD_CHECK(hipMemcpy(L.items.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
