@@ -112,6 +112,43 @@ int mvd_destroy(mvd_session* s);
  * be odd (cuda/BlockGeneratorFixedSizePrecise.java:43). */
 int mvd_add_view(mvd_session* s, const float* img, const float* weight, const float* psf, int kx, int ky, int kz);
 
+/* Input preparation of one view, on the GPU: the loop body of
+ * ProcessForDeconvolution (deconv/ProcessForDeconvolution.java:187-277) with
+ * TransformInputAndWeights (deconv/TransformInputAndWeights.java:100-135) or
+ * TransformInput (deconv/TransformInput.java:69-95), then MVDeconFFT + add as
+ * mvd_add_view.  raw: the view as acquired, rx*ry*rz floats, x-fastest.
+ * inverse_affine: 12 doubles, the row-packed 3x4 inverse of the view's
+ * registration (AffineTransform3D.inverse().getRowPackedCopy()), mapping
+ * bounding-box coordinates to raw-view coordinates.  offset: the bounding
+ * box's min (x, y, z); the session's image size is the bounding box's size.
+ * The view becomes max(minValue, trilinear interpolation over the
+ * mirror-single extension) where the mapped position lies inside the raw view
+ * and 0 elsewhere.  border and range (x, y, z; both or neither): the view's
+ * weight becomes its blending weight (BlendingRealRandomAccess.java:91-121)
+ * at the mapped position -- PRECOMPUTED weights, normalised later by
+ * mvd_set_weights; NULL: no weight (w = 1). */
+int mvd_add_view_transformed(mvd_session* s, const float* raw, int rx, int ry, int rz, const double* inverse_affine,
+                             const long long* offset, const float* border, const float* range, const float* psf,
+                             int kx, int ky, int kz);
+
+/* The same resampling without a session (host arrays in and out), for callers
+ * that replace TransformInputAndWeights / TransformInput / TransformWeights
+ * (deconv/TransformWeights.java:53-91) one view at a time.  out_img (nullable;
+ * raw may then be NULL) receives nx*ny*nz floats as above; out_weight
+ * (nullable) the blending weight, which needs border and range. */
+int mvd_transform_view(int device, const float* raw, int rx, int ry, int rz, const double* inverse_affine,
+                       const long long* offset, int nx, int ny, int nz, const float* border, const float* range,
+                       float* out_img, float* out_weight);
+
+/* mvd_transform_view on device memory: raw, out_img and out_weight are device
+ * pointers on `device`, the launch is queued on `stream` (a hipStream_t; NULL =
+ * the default stream) and the call returns without waiting.  No counterpart in
+ * the reference, whose views are host images: for callers that keep the views
+ * in HBM (a resampling step fused into a GPU pipeline). */
+int mvd_transform_view_device(int device, const float* raw, int rx, int ry, int rz, const double* inverse_affine,
+                              const long long* offset, int nx, int ny, int nz, const float* border,
+                              const float* range, float* out_img, float* out_weight, void* stream);
+
 /* Weight handling before preparation (ProcessForDeconvolution.java:325-405):
  * mode MVD_WEIGHTS_*; osem_index as EfficientBayesianBased.osemspeedupIndex
  * (deconv/EfficientBayesianBased.java:1023-1042, ProcessForDeconvolution.java:353-356):

@@ -287,3 +287,59 @@ def nll_psf_fd(view, psf, psi, voxels, h=1e-3, weight=None, threads=None):
     _check(L.orc_nll_view_fd_f64(_ptr(v), None if wt is None else _ptr(wt), _ptr(blur), _ptr(s), _ptr(_dims(s)),
                                  _ptr(_dims(p)), _ptr(vox), len(vox), h, _ptr(out), threads or default_threads()))
     return out
+
+
+def _declare_transform(L):
+    L.orc_blend_lut.argtypes = [_P]
+    L.orc_blend_lut.restype = None
+    L.orc_blend_weight.argtypes = [_P, _P, _P, _P]
+    L.orc_blend_weight.restype = ctypes.c_float
+    L.orc_transform_view.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int]
+
+
+def blend_lut():
+    """The blending lookup table (1001 doubles)."""
+    L = lib()
+    _declare_transform(L)
+    out = np.empty(1001, np.float64)
+    L.orc_blend_lut(_ptr(out))
+    return out
+
+
+def blend_weight(t_xyz, size_xyz, border_xyz, range_xyz) -> np.float32:
+    """Blending weight of a view of size_xyz at one (float) position."""
+    L = lib()
+    _declare_transform(L)
+    t = np.asarray(t_xyz, np.float32)
+    return np.float32(L.orc_blend_weight(_ptr(t), _ptr(np.asarray(size_xyz, np.int64)),
+                                         _ptr(np.asarray(border_xyz, np.float32)),
+                                         _ptr(np.asarray(range_xyz, np.float32))))
+
+
+def transform_view(raw, inverse, offset, size_xyz, border=None, rng=None, want_image=True, raw_size_xyz=None,
+                   threads=None):
+    """One view resampled into the bounding box (size_xyz, min = offset) by
+    the row-packed 3x4 inverse map.  Returns (image or None, weight or None);
+    the image is 0 where the position falls outside the raw view."""
+    L = lib()
+    _declare_transform(L)
+    inv = np.ascontiguousarray(np.asarray(inverse, np.float64).reshape(12))
+    off = np.ascontiguousarray(np.asarray(offset, np.int64).reshape(3))
+    nx, ny, nz = (int(v) for v in size_xyz)
+    if raw is not None:
+        raw = np.ascontiguousarray(raw, np.float32)
+        rs = np.array([raw.shape[2], raw.shape[1], raw.shape[0]], np.int64)
+    else:
+        rs = np.asarray(raw_size_xyz, np.int64)
+    img = np.zeros((nz, ny, nx), np.float32) if want_image else None
+    w = None
+    b = r = None
+    if border is not None:
+        b = np.asarray(border, np.float32)
+        r = np.asarray(rng, np.float32)
+        w = np.empty((nz, ny, nx), np.float32)
+    _check(L.orc_transform_view(None if raw is None else _ptr(raw), _ptr(rs), _ptr(inv), _ptr(off),
+                                _ptr(np.array([nx, ny, nz], np.int64)), None if b is None else _ptr(b),
+                                None if r is None else _ptr(r), None if img is None else _ptr(img),
+                                None if w is None else _ptr(w), threads or default_threads()))
+    return img, w

@@ -19,7 +19,9 @@ EXPORTS = (
     "getNumDevicesCUDA", "getNameDeviceCUDA", "getMemDeviceCUDA", "getFreeMemDeviceCUDA",
     "getCUDAcomputeCapabilityMajorVersion", "getCUDAcomputeCapabilityMinorVersion",
     "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
-    "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
+    "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_add_view_transformed", "mvd_transform_view",
+    "mvd_transform_view_device",
+    "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
     "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
     "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_nll_grad_device",
     "mvd_get_grad_psi", "mvd_set_fft_policy",
@@ -50,6 +52,12 @@ def _declare(L):
     L.mvd_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
     L.mvd_destroy.argtypes = [_P]
     L.mvd_add_view.argtypes = [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.mvd_add_view_transformed.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.mvd_transform_view.argtypes = [ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
+    L.mvd_transform_view_device.argtypes = [ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]
     L.mvd_set_weights.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _P]
     L.mvd_prepare.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, ctypes.c_int]
     L.mvd_init_psi.argtypes = [_P, _P, _P]
@@ -82,6 +90,14 @@ def lib():
             raise NativeError(
                 f"{LIB_PATH} is missing: build it with `python -m spim_registration_amd.build` "
                 "(there is no CPU fallback)")
+        # torch ships its own HIP runtime under the soname this library links
+        # (libamdhip64.so.7); loaded first, it is the one copy both use.  Loaded
+        # after this library, torch would bring a second runtime that cannot
+        # open the device, so torch is imported first where it is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         try:
             L = ctypes.CDLL(LIB_PATH)
         except OSError as e:

@@ -269,6 +269,26 @@ hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double
 hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
 hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream);
 
+// Resampling of one view into the bounding box with its blending weight
+// (transform.hip).  inv: row-packed 3x4 inverse affine map (bounding-box
+// coordinates -> raw view); off: bounding-box min as float; out_img / out_w
+// nullable; lut: the blending table in device memory (blend_lut_device).
+constexpr int kBlendLutSize = 1001;
+struct TransformArgs {
+  const float* raw;
+  int rsize[3];
+  double inv[12];
+  float off[3];
+  int nx, ny, nz;
+  float border[3], range[3];
+  float* out_img;
+  float* out_w;
+  const double* lut;
+};
+const double* blend_lut_device(int device);  // per device, held until the process ends
+const char* check_transform_args(const TransformArgs& a);  // nullptr when valid
+hipError_t launch_transform_view(const TransformArgs& a, hipStream_t stream);
+
 // Plan cache and kernel-spectrum helpers (session.cpp).
 void set_error(const std::string& msg);
 

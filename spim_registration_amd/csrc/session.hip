@@ -190,13 +190,14 @@ enum TimingClass {
   kTXYInvQuotient,
   kTXYInvUpdate,
   kTXYInvStore,
+  kTTransform,
   kTNumClasses
 };
 static const char* kTimingNames[kTNumClasses] = {"x_forward",         "y_forward",       "z_conv",
                                                  "y_inverse",         "x_inverse_quotient", "x_inverse_update",
                                                  "x_inverse_store",   "z_forward",       "reduce_stats",
                                                  "pointwise",         "xy_forward",      "xy_inverse_quotient",
-                                                 "xy_inverse_update", "xy_inverse_store"};
+                                                 "xy_inverse_update", "xy_inverse_store", "transform_view"};
 
 class Timer {
  public:
@@ -1081,12 +1082,27 @@ int mvd_destroy(mvd_session* s) {
   return 0;
 }
 
-int mvd_add_view(mvd_session* s, const float* img, const float* weight, const float* psf, int kx, int ky, int kz) {
-  MVD_REQUIRE(s && img && psf, "NULL argument");
+static int check_view_args(mvd_session* s, const float* psf, int kx, int ky, int kz) {
+  MVD_REQUIRE(s && psf, "NULL argument");
   MVD_REQUIRE((int)s->views.size() < s->nviews, "all views already added");
   MVD_REQUIRE(kx > 0 && ky > 0 && kz > 0 && (kx & 1) && (ky & 1) && (kz & 1),
               "kernel sizes must be odd (BlockGeneratorFixedSizePrecise)");
   MVD_REQUIRE(kx <= s->img.x + 4096 && ky <= s->img.y + 4096, "kernel too large");
+  return 0;
+}
+
+static void register_view(mvd_session* s, const float* psf, int kx, int ky, int kz, bool has_weight) {
+  s->views.emplace_back();
+  auto& view = s->views.back();
+  view.kd = {kx, ky, kz};
+  view.psf_raw.assign(psf, psf + (size_t)kx * ky * kz);
+  view.has_weight = has_weight;
+  s->prepared = false;
+}
+
+int mvd_add_view(mvd_session* s, const float* img, const float* weight, const float* psf, int kx, int ky, int kz) {
+  if (int rc = check_view_args(s, psf, kx, ky, kz)) return rc;
+  MVD_REQUIRE(img, "NULL argument");
   DeviceGuard guard(s->device);
   const int v = (int)s->views.size();
   if (weight) {
@@ -1097,12 +1113,50 @@ int mvd_add_view(mvd_session* s, const float* img, const float* weight, const fl
   MVD_CHECK(hipMemcpyAsync(s->d_views.as<float>() + v * s->n, img, sizeof(float) * s->n, hipMemcpyHostToDevice,
                            s->stream));
   MVD_CHECK(hipStreamSynchronize(s->stream));
-  s->views.emplace_back();
-  auto& view = s->views.back();
-  view.kd = {kx, ky, kz};
-  view.psf_raw.assign(psf, psf + (size_t)kx * ky * kz);
-  view.has_weight = weight != nullptr;
-  s->prepared = false;
+  register_view(s, psf, kx, ky, kz, weight != nullptr);
+  return 0;
+}
+
+int mvd_add_view_transformed(mvd_session* s, const float* raw, int rx, int ry, int rz, const double* inverse_affine,
+                             const long long* offset, const float* border, const float* range, const float* psf,
+                             int kx, int ky, int kz) {
+  if (int rc = check_view_args(s, psf, kx, ky, kz)) return rc;
+  MVD_REQUIRE(raw && inverse_affine && offset, "NULL argument");
+  MVD_REQUIRE((border == nullptr) == (range == nullptr), "weights need both border and range");
+  DeviceGuard guard(s->device);
+  const int v = (int)s->views.size();
+  TransformArgs a{};
+  a.rsize[0] = rx;
+  a.rsize[1] = ry;
+  a.rsize[2] = rz;
+  for (int i = 0; i < 12; ++i) a.inv[i] = inverse_affine[i];
+  for (int d = 0; d < 3; ++d) a.off[d] = (float)offset[d];
+  a.nx = s->img.x;
+  a.ny = s->img.y;
+  a.nz = s->img.z;
+  a.out_img = s->d_views.as<float>() + v * s->n;
+  if (border) {
+    MVD_CHECK(s->d_weights.ensure(sizeof(float) * s->n * s->nviews));
+    for (int d = 0; d < 3; ++d) {
+      a.border[d] = border[d];
+      a.range[d] = range[d];
+    }
+    a.out_w = s->d_weights.as<float>() + v * s->n;
+    a.lut = blend_lut_device(s->device);
+    MVD_REQUIRE(a.lut, "blending table allocation failed");
+  }
+  a.raw = raw;  // checked for presence; replaced by the device copy below
+  if (const char* msg = check_transform_args(a)) MVD_REQUIRE(false, msg);
+  DeviceBuf d_raw;
+  MVD_CHECK(d_raw.ensure(sizeof(float) * (size_t)rx * ry * rz));
+  MVD_CHECK(hipMemcpyAsync(d_raw.p, raw, sizeof(float) * (size_t)rx * ry * rz, hipMemcpyHostToDevice, s->stream));
+  a.raw = d_raw.as<float>();
+  s->timer.begin(s->stream);
+  MVD_CHECK(launch_transform_view(a, s->stream));
+  s->timer.end(s->stream, kTTransform);
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  s->timer.collect();
+  register_view(s, psf, kx, ky, kz, border != nullptr);
   return 0;
 }
 

@@ -78,12 +78,56 @@ class MVDeconFFT:
         self.num_views = 0
         self._session = None
         self._index = -1
+        self._raw = None  # (raw view, inverse map, offset, blending) of a view resampled on the GPU
 
-    # getters of the reference (MVDeconFFT.java:371-377)
+    @property
+    def shape(self):
+        return self.image.shape if self._raw is None else self._raw_shape
+
+    @classmethod
+    def from_raw(cls, raw, inverse, offset: Sequence[int], size_xyz: Sequence[int], kernel, blending=None,
+                 device_list: Sequence[int] = (0,), use_blocks: bool = False,
+                 block_size: Optional[Sequence[int]] = None) -> "MVDeconFFT":
+        """A view given as acquired: resampled into the bounding box (size_xyz,
+        min = offset) on the GPU when the deconvolution adds it, with its
+        blending weight when `blending` (a transform.Blending) is given
+        (ProcessForDeconvolution.java:187-277; TransformInputAndWeights /
+        TransformInput).  `inverse`: row-packed 3x4 inverse registration."""
+        from .transform import _inv12
+        nx, ny, nz = (int(v) for v in size_xyz)
+        placeholder = np.zeros((1, 1, 1), np.float32)
+        v = cls(placeholder, None, kernel, device_list, use_blocks, block_size)
+        v.image = None
+        v._raw = (np.ascontiguousarray(raw, np.float32), _inv12(inverse),
+                  np.ascontiguousarray(np.asarray(offset, np.int64).reshape(3)), blending)
+        if v._raw[0].ndim != 3:
+            raise ValueError("expected a 3D [z, y, x] raw view")
+        v._raw_shape = (nz, ny, nx)
+        v._has_blending = blending is not None
+        return v
+
+    def has_weight(self) -> bool:
+        return self.weight is not None if self._raw is None else self._has_blending
+
+    def _resample(self):
+        from .transform import transform_view
+        raw, inv, off, blending = self._raw
+        nz, ny, nx = self._raw_shape
+        dev = self.device_list[0]
+        return transform_view(raw, inv, off, (nx, ny, nz), blending, device=dev)
+
+    # getters of the reference (MVDeconFFT.java:371-377); a view given raw is
+    # resampled on the GPU on first access
     def get_image(self):
+        if self.image is None and self._raw is not None:
+            self.image, w = self._resample()
+            if self.weight is None:
+                self.weight = w
         return self.image
 
     def get_weight(self):
+        if self.weight is None and self._raw is not None and self._has_blending:
+            self.get_image()
         return self.weight
 
     def get_kernel1(self):
@@ -142,8 +186,18 @@ class _Session:
 
     def add(self, view: MVDeconFFT):
         kz, ky, kx = view.kernel1.shape
-        _native.check(_native.lib().mvd_add_view(self.handle, _ptr(view.image), _ptr(view.weight),
-                                                 _ptr(view.kernel1), kx, ky, kz), "mvd_add_view")
+        if view._raw is not None:
+            raw, inv, off, blending = view._raw
+            rz, ry, rx = raw.shape
+            b = r = None
+            if blending is not None:
+                b, r = blending.arrays()
+            _native.check(_native.lib().mvd_add_view_transformed(
+                self.handle, _ptr(raw), rx, ry, rz, _ptr(inv), _ptr(off), _ptr(b), _ptr(r), _ptr(view.kernel1),
+                kx, ky, kz), "mvd_add_view_transformed")
+        else:
+            _native.check(_native.lib().mvd_add_view(self.handle, _ptr(view.image), _ptr(view.weight),
+                                                     _ptr(view.kernel1), kx, ky, kz), "mvd_add_view")
         view._session = self
         view._index = len(self.views)
         self.views.append(view)
@@ -182,9 +236,9 @@ class MVDeconvolution:
         vs = views.get_views()
         if not vs:
             raise ValueError("no views")
-        shape = vs[0].image.shape
+        shape = vs[0].shape
         for v in vs:
-            if v.image.shape != shape:
+            if v.shape != shape:
                 raise ValueError("all views must have the same size")
         v0 = vs[0]
         self.name = name
@@ -197,7 +251,7 @@ class MVDeconvolution:
         for v in vs:
             self._s.add(v)
         if weight_type is None:
-            weight_type = WeightType.AS_GIVEN if all(v.weight is not None for v in vs) else WeightType.NO_WEIGHTS
+            weight_type = WeightType.AS_GIVEN if all(v.has_weight() for v in vs) else WeightType.NO_WEIGHTS
         self.weight_type = WeightType(weight_type)
         mm = (ctypes.c_double * 2)()
         L = _native.lib()
