@@ -163,11 +163,16 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P0.tw)[i];
   const int col = blockIdx.x * L + line;
-  const bool valid = col < P0.hx;
   const uint32_t plane = (uint32_t)(P0.qy * P0.pitch) * 8u;  // bytes
   const uint32_t bytes = plane * (uint32_t)N;
-  // lanes past the last column load a valid column and discard it
-  const uint32_t off = (uint32_t)(blockIdx.y * P0.pitch + (valid ? col : P0.hx - 1)) * 8u;
+  // Lanes of the padding columns [hx, pitch) transform their own (unused)
+  // column and store it, so that every 128-byte line of a row is written whole
+  // (a partly written line reaches HBM as a read-modify-write).  Lanes past the
+  // pitch read column hx - 1 and store nothing.  A line's transform never
+  // mixes with other lines.
+  const uint32_t row = (uint32_t)(blockIdx.y * P0.pitch) * 8u;  // uniform: the SGPR offset
+  const bool store_col = col < P0.pitch;
+  const uint32_t off = (uint32_t)(store_col ? col : P0.hx - 1) * 8u;
   auto addr = [line](int i) { return i * LP + line; };
   auto load_line = [&](const float2* spec, float2(&v)[E]) {
     const BufRes r = buf_res(spec, bytes);
@@ -176,7 +181,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
-        v[b * S::R0 + t] = buf_load2(r, z * plane + off);
+        v[b * S::R0 + t] = buf_load2(r, z * plane + off, row);
       }
   };
   float2 nxt[E];
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const uint32_t z = elem_index<N, E, RL>(jt, b, t);
-        kv[b * RL + t] = buf_load2(r, z * plane + off);
+        kv[b * RL + t] = buf_load2(r, z * plane + off, row);
       }
   }
   // wait for the kernel spectrum here, ahead of the loop and its prefetches
@@ -215,8 +220,8 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const int z = elem_index<N, E, S::R0>(jt, b, t);
-        const bool keep = valid && z >= zlo && z < zhi;
-        buf_store2(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t]);
+        const bool keep = store_col && z >= zlo && z < zhi;
+        buf_store2(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t], row);
       }
   }
 }

@@ -126,20 +126,25 @@ typedef __amdgpu_buffer_rsrc_t BufRes;
 __device__ __forceinline__ BufRes buf_res(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t byte_off) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+// soff: a wave-uniform byte offset (held in an SGPR) added to the lane's.
+__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t byte_off, uint32_t soff = 0) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, soff, 0));
 }
-__device__ __forceinline__ void buf_store2(BufRes r, uint32_t byte_off, float2 v) {
+__device__ __forceinline__ void buf_store2(BufRes r, uint32_t byte_off, float2 v, uint32_t soff = 0) {
   typedef unsigned int U2 __attribute__((vector_size(8)));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, byte_off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, byte_off, soff, 0);
 }
 #else
 struct BufRes {
   char* p;
 };
 __device__ __forceinline__ BufRes buf_res(const void* base, uint32_t) { return BufRes{(char*)base}; }
-__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t o) { return *(const float2*)(r.p + o); }
-__device__ __forceinline__ void buf_store2(BufRes r, uint32_t o, float2 v) { *(float2*)(r.p + o) = v; }
+__device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t o, uint32_t so = 0) {
+  return *(const float2*)(r.p + o + so);
+}
+__device__ __forceinline__ void buf_store2(BufRes r, uint32_t o, float2 v, uint32_t so = 0) {
+  *(float2*)(r.p + o + so) = v;
+}
 #endif
 
 FftPlan make_plan(int n);

@@ -644,6 +644,7 @@ int build_block_batch(mvd_session* s) {
   std::vector<float2*> spec;
   for (auto& b : bb.spec) {
     MVD_CHECK(b.ensure(sizeof(float2) * elems));
+    MVD_CHECK(hipMemsetAsync(b.p, 0, sizeof(float2) * elems, s->stream));  // finite row padding
     spec.push_back(b.as<float2>());
   }
   bb.view.resize(s->nviews);
@@ -712,6 +713,9 @@ int run_conv(mvd_session* s, const ShapePlan& sp, const ConvGeom& g, const float
 // Spectrum of a kernel wrapped into the FFT box of sp, scaled by 1/(qx qy qz).
 int kernel_spectrum(mvd_session* s, const ShapePlan& sp, const std::vector<float>& k, Int3 kd, DeviceBuf* out) {
   MVD_CHECK(out->ensure(sizeof(float2) * sp.spectrum_elems()));
+  // the row padding [hx, pitch) is never written by the transforms; zeros
+  // there keep the (discarded) padding lanes of the z pass finite
+  MVD_CHECK(hipMemsetAsync(out->p, 0, sizeof(float2) * sp.spectrum_elems(), s->stream));
   DeviceBuf dk;
   MVD_CHECK(dk.ensure(sizeof(float) * k.size()));
   MVD_CHECK(hipMemcpyAsync(dk.p, k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s->stream));
@@ -1718,6 +1722,8 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph) {
       for (auto& v : s->views)
         if (i < v.blocks.size()) elems = std::max<size_t>(elems, v.plan->spectrum_elems());
       MVD_CHECK(b->spec.back().back().ensure(sizeof(float2) * elems));
+      // finite row padding (before any capture)
+      MVD_CHECK(hipMemsetAsync(b->spec.back().back().p, 0, sizeof(float2) * elems, b->stream));
     }
   }
   b->total_stats = tot;
