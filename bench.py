@@ -388,6 +388,60 @@ def run_resample(args):
     print(json.dumps(out), flush=True)
 
 
+def run_psf(args):
+    """PSF extraction (ExtractPSF.extractNextImg) on the GPU: 2000 beads in a
+    512x512x128 view, PSF box 41x41x81 (the 1024x1024x512 workload's), model =
+    rotation by 45 deg about y with z spacing 2.  A step is one mvd_extract_psf
+    call with host arrays (image upload, extraction, normalisation, transform,
+    download), so the rate includes PCIe."""
+    import math
+    import torch
+    from oracle import oracle as O
+    from spim_registration_amd import psf as P
+    from spim_registration_amd.transform import inverse_affine
+
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 512, 512, 128
+    img = rng.uniform(100.0, 200.0, (nz, ny, nx)).astype(np.float32)
+    nb = 2000
+    locs = np.stack([rng.uniform(0, nx - 1, nb), rng.uniform(0, ny - 1, nb), rng.uniform(0, nz - 1, nb)], 1)
+    size = (41, 41, 81)
+    a = math.radians(45.0)
+    r = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]]) @ np.diag([1, 1, 2.0])
+    model = np.hstack([r, np.zeros((3, 1))]).reshape(12)
+    for _ in range(args.warmup):
+        P.extract_psf(img, locs, size, model)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        psf, tpsf = P.extract_psf(img, locs, size, model)
+    sec = (time.perf_counter() - t0) / args.steps
+    m = int(np.prod(size))
+    value = m * nb / sec / 1e9
+    log(f"{args.steps} extractions, {1e3 * sec:.2f} ms each -> {value:.2f} G interpolations/s (PCIe included)")
+    _, rpsf = O.extract_psf(img, locs[:50], size)
+    psf50, _ = P.extract_psf(img, locs[:50], size, None)
+    exact = bool(np.array_equal(psf50, rpsf))
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        O.extract_psf(img, locs[:50], size)
+        cs = time.perf_counter() - t0
+        cpu = dict(value=round(m * 50 / cs / 1e9, 4), unit="G interpolations/s", cores=1, kind="port",
+                   sample=f"oracle C++ extraction of 50 beads (41x41x81 box), single thread, {cs:.2f} s")
+    out = {
+        "metric": "PSF extraction: bead x PSF-voxel interpolations per second (host arrays, PCIe included)",
+        "value": round(value, 3), "unit": "G interpolations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * sec, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 positions and weights)", "data": "synthetic",
+        "config": {"workload": "psf: 2000 beads in a 512x512x128 view, PSF 41x41x81, model rot 45 deg + z x2",
+                   "transformed_size": list(tpsf.shape[::-1])},
+        "parity": {"beads": 50, "bit_exact_vs_oracle": exact},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +472,9 @@ def main():
         return
     if args.workload == "resample":
         run_resample(args)
+        return
+    if args.workload == "psf":
+        run_psf(args)
         return
     from spim_registration_amd import MVDeconFFT, MVDeconInput, MVDeconvolution, PSFTYPE, WeightType
     from spim_registration_amd import synthetic as S

@@ -95,6 +95,26 @@ enum {
   MVD_WEIGHTS_VIRTUAL = 3      /* w_v / max(sum, 1) evaluated per voxel (NormalizingRandomAccess) */
 };
 
+/* PSF extraction from bead locations, on the GPU: ExtractPSF.extractNextImg
+ * (deconv/ExtractPSF.java:277-296) = extractPSFLocal (:374-415: the image,
+ * extended periodically, interpolated trilinearly at every PSF voxel around
+ * every bead and summed in bead order) + normalize (:298-317: to [0, 1]) +
+ * transformPSF (:325-372: resampled by the view's model into an odd box whose
+ * centre is the transformed centre, zero out of bounds).
+ * img: nx*ny*nz floats (the view as acquired); locations: nloc x (x, y, z)
+ * doubles in pixel units; psf_size: (x, y, z).  psf_out receives the
+ * normalised PSF in input calibration (getInputCalibrationPSFs);
+ * transformed_out (nullable) the transformed PSF (getTransformedPSF), of the
+ * size mvd_psf_transform_size returns; it needs the forward model and its
+ * inverse (row-packed 3x4: AffineTransform3D.getRowPackedCopy() of the model
+ * and of model.inverse()). */
+int mvd_extract_psf(int device, const float* img, int nx, int ny, int nz, const double* locations, int nloc,
+                    const int* psf_size, const double* model, const double* inverse_model, float* psf_out,
+                    float* transformed_out);
+/* Size (x, y, z) of the transformed PSF for a PSF of psf_size and the forward
+ * model (ExtractPSF.transformPSF, deconv/ExtractPSF.java:325-372). */
+int mvd_psf_transform_size(const int* psf_size, const double* model, int* new_size);
+
 /* FFT shape of each block window (call before mvd_prepare). */
 enum {
   MVD_FFT_SMOOTH = 0,   /* smallest 5-smooth size per axis (default) */

@@ -343,3 +343,50 @@ def transform_view(raw, inverse, offset, size_xyz, border=None, rng=None, want_i
                                 None if r is None else _ptr(r), None if img is None else _ptr(img),
                                 None if w is None else _ptr(w), threads or default_threads()))
     return img, w
+
+
+def _declare_psf(L):
+    L.orc_extract_psf.argtypes = [_P, _P, _P, ctypes.c_int64, _P, _P, _P]
+    L.orc_psf_transform_geometry.argtypes = [_P, _P, _P, _P]
+    L.orc_psf_transform_geometry.restype = None
+    L.orc_psf_transform.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.orc_psf_transform.restype = None
+
+
+def extract_psf(img, locations, size_xyz):
+    """extractPSFLocal + normalize: (raw sum, normalised PSF), float32 [z, y, x]."""
+    L = lib()
+    _declare_psf(L)
+    img = np.ascontiguousarray(img, np.float32)
+    loc = np.ascontiguousarray(np.asarray(locations, np.float64).reshape(-1, 3))
+    sx, sy, sz = (int(v) for v in size_xyz)
+    raw = np.empty((sz, sy, sx), np.float32)
+    out = np.empty_like(raw)
+    _check(L.orc_extract_psf(_ptr(img), _ptr(_dims(img)), _ptr(loc), len(loc),
+                             _ptr(np.array([sx, sy, sz], np.int64)), _ptr(raw), _ptr(out)))
+    return raw, out
+
+
+def psf_transform_geometry(size_xyz, model):
+    L = lib()
+    _declare_psf(L)
+    ns = np.empty(3, np.int64)
+    off = np.empty(3, np.float64)
+    L.orc_psf_transform_geometry(_ptr(np.asarray(size_xyz, np.int64)),
+                                 _ptr(np.ascontiguousarray(np.asarray(model, np.float64).reshape(12))), _ptr(ns),
+                                 _ptr(off))
+    return tuple(int(v) for v in ns), off
+
+
+def psf_transform(psf, model, inverse):
+    """transformPSF: the PSF resampled by the forward model into its odd box."""
+    L = lib()
+    _declare_psf(L)
+    psf = np.ascontiguousarray(psf, np.float32)
+    size = (psf.shape[2], psf.shape[1], psf.shape[0])
+    ns, off = psf_transform_geometry(size, model)
+    out = np.empty((ns[2], ns[1], ns[0]), np.float32)
+    L.orc_psf_transform(_ptr(psf), _ptr(np.asarray(size, np.int64)),
+                        _ptr(np.ascontiguousarray(np.asarray(inverse, np.float64).reshape(12))),
+                        _ptr(np.asarray(ns, np.int64)), _ptr(off), _ptr(out))
+    return out

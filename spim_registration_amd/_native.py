@@ -20,7 +20,7 @@ EXPORTS = (
     "getCUDAcomputeCapabilityMajorVersion", "getCUDAcomputeCapabilityMinorVersion",
     "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_add_view_transformed", "mvd_transform_view",
-    "mvd_transform_view_device",
+    "mvd_transform_view_device", "mvd_extract_psf", "mvd_psf_transform_size",
     "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
     "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
     "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_nll_grad_device",
@@ -58,6 +58,9 @@ def _declare(L):
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
     L.mvd_transform_view_device.argtypes = [ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P]
+    L.mvd_extract_psf.argtypes = [ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P,
+                                  _P, _P, _P, _P]
+    L.mvd_psf_transform_size.argtypes = [_P, _P, _P]
     L.mvd_set_weights.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _P]
     L.mvd_prepare.argtypes = [_P, ctypes.c_int, ctypes.c_double, _P, ctypes.c_int]
     L.mvd_init_psi.argtypes = [_P, _P, _P]
