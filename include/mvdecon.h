@@ -74,6 +74,30 @@ void convolution3DfftCUDAInPlace(float* im, int* imDim, float* kernel, int* kern
 float* convolution3DfftCUDA(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA);
 void mvd_free(void* p);
 
+/* Replace CUDASeparableConvolution.convolve_{7,15,31,63,127}
+ * (spim/process/cuda/CUDASeparableConvolution.java:34-38), the second native
+ * hook (difference-of-Gaussian detection, via
+ * CUDASeparableConvolutionFunctions.gauss, :123-217): in place, image of
+ * imageW x imageH x imageD floats (x fastest), kernels of N floats centred at
+ * N/2 (used where convolveX/Y/Z != 0, in that order); outofbounds 0 = zero,
+ * 1 = outofboundsvalue, 2 = nearest border pixel.  Returns 1 on success, 0 on
+ * failure (printed to stderr, image unchanged). */
+int convolve_7(float* image, float* kernelX, float* kernelY, float* kernelZ, int imageW, int imageH, int imageD,
+               int convolveX, int convolveY, int convolveZ, int outofbounds, float outofboundsvalue, int devCUDA);
+int convolve_15(float* image, float* kernelX, float* kernelY, float* kernelZ, int imageW, int imageH, int imageD,
+                int convolveX, int convolveY, int convolveZ, int outofbounds, float outofboundsvalue, int devCUDA);
+int convolve_31(float* image, float* kernelX, float* kernelY, float* kernelZ, int imageW, int imageH, int imageD,
+                int convolveX, int convolveY, int convolveZ, int outofbounds, float outofboundsvalue, int devCUDA);
+int convolve_63(float* image, float* kernelX, float* kernelY, float* kernelZ, int imageW, int imageH, int imageD,
+                int convolveX, int convolveY, int convolveZ, int outofbounds, float outofboundsvalue, int devCUDA);
+int convolve_127(float* image, float* kernelX, float* kernelY, float* kernelZ, int imageW, int imageH, int imageD,
+                 int convolveX, int convolveY, int convolveZ, int outofbounds, float outofboundsvalue, int devCUDA);
+/* Replaces CUDASeparableConvolution.convolutionCPU (:42): kernels of
+ * 2 * kernelR + 1 taps (NULL = axis skipped).  Runs the same passes on GPU 0
+ * (this library has no CPU path); errors are printed. */
+void convolutionCPU(float* image, float* kernelX, float* kernelY, float* kernelZ, int kernelRX, int kernelRY,
+                    int kernelRZ, int imageW, int imageH, int imageD, int outofbounds, float outofboundsvalue);
+
 /* ---- 2. Session API ----------------------------------------------------- */
 
 typedef struct mvd_session mvd_session;

@@ -19,6 +19,7 @@ EXPORTS = (
     "getNumDevicesCUDA", "getNameDeviceCUDA", "getMemDeviceCUDA", "getFreeMemDeviceCUDA",
     "getCUDAcomputeCapabilityMajorVersion", "getCUDAcomputeCapabilityMinorVersion",
     "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
+    "convolve_7", "convolve_15", "convolve_31", "convolve_63", "convolve_127", "convolutionCPU",
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_add_view_transformed", "mvd_transform_view",
     "mvd_transform_view_device", "mvd_extract_psf", "mvd_psf_transform_size",
     "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
@@ -47,6 +48,12 @@ def _declare(L):
     L.convolution3DfftCUDAInPlace.restype = None
     L.convolution3DfftCUDA.argtypes = [_P, _P, _P, _P, ctypes.c_int]
     L.convolution3DfftCUDA.restype = ctypes.POINTER(ctypes.c_float)
+    for n in (7, 15, 31, 63, 127):
+        f = getattr(L, f"convolve_{n}")
+        f.argtypes = [_P, _P, _P, _P] + [ctypes.c_int] * 7 + [ctypes.c_float, ctypes.c_int]
+        f.restype = ctypes.c_int
+    L.convolutionCPU.argtypes = [_P, _P, _P, _P] + [ctypes.c_int] * 7 + [ctypes.c_float]
+    L.convolutionCPU.restype = None
     L.mvd_free.argtypes = [_P]
     L.mvd_free.restype = None
     L.mvd_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]

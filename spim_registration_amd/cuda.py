@@ -158,3 +158,79 @@ def convolve_blocks_jna(cuda: CUDAFourierConvolution, image: np.ndarray, kernel:
         cuda.convolution3DfftCUDAInPlace(flat, np.array([bz, by, bx], np.int32), kflat, kdim, device)
         b.paste_block(result, flat.reshape(bz, by, bx))
     return result
+
+
+class CUDASeparableConvolution(CUDAFourierConvolution):
+    """What `Native.load(path, CUDASeparableConvolution.class)` returns in the
+    reference (CUDASeparableConvolution.java:30-43): in-place separable
+    convolution with kernels of N in {7, 15, 31, 63, 127} taps."""
+
+    def _convolve(self, n, image, kernelX, kernelY, kernelZ, imageW, imageH, imageD, convolveX, convolveY,
+                  convolveZ, outofbounds, outofboundsvalue, devCUDA) -> bool:
+        if image.dtype != np.float32 or not image.flags["C_CONTIGUOUS"]:
+            raise TypeError("image must be a contiguous float32 array (Java float[])")
+        if image.size != imageW * imageH * imageD:
+            raise ValueError("array length does not match its dimensions")
+        p = ctypes.c_void_p
+        ks = []
+        for k, on in ((kernelX, convolveX), (kernelY, convolveY), (kernelZ, convolveZ)):
+            if k is None:
+                ks.append(None)
+                continue
+            k = np.ascontiguousarray(k, dtype=np.float32)
+            if on and k.size != n:
+                raise ValueError(f"kernels of convolve_{n} have {n} taps")
+            ks.append(k)
+        f = getattr(self._L, f"convolve_{n}")
+        return bool(f(image.ctypes.data_as(p), *[None if k is None else k.ctypes.data_as(p) for k in ks],
+                      imageW, imageH, imageD, int(bool(convolveX)), int(bool(convolveY)), int(bool(convolveZ)),
+                      int(outofbounds), float(outofboundsvalue), devCUDA))
+
+    def convolve_7(self, *a) -> bool:
+        return self._convolve(7, *a)
+
+    def convolve_15(self, *a) -> bool:
+        return self._convolve(15, *a)
+
+    def convolve_31(self, *a) -> bool:
+        return self._convolve(31, *a)
+
+    def convolve_63(self, *a) -> bool:
+        return self._convolve(63, *a)
+
+    def convolve_127(self, *a) -> bool:
+        return self._convolve(127, *a)
+
+
+SUPPORTED_KERNEL_SIZES = (7, 15, 31, 63, 127)
+OOB_ZERO_SEP, OOB_VALUE_SEP, OOB_EXTEND_BORDER_PIXELS = 0, 1, 2
+
+
+def gauss(img: np.ndarray, dim: Sequence[int], kernels: Sequence[np.ndarray], oobs: int = OOB_EXTEND_BORDER_PIXELS,
+          oobs_value: float = 0.0, cuda: CUDASeparableConvolution = None, device: int = 0) -> bool:
+    """CUDASeparableConvolutionFunctions.gauss (:123-217) given the per-axis
+    kernels (1 to 3, x first): every kernel is centred in zeros of the
+    smallest supported size that holds the longest one
+    (getCUDAKernels / getFloatKernelPadded, :219-258), then convolve_N."""
+    n = len(dim)
+    if n == 0 or n > 3 or len(kernels) != n:
+        return False
+    longest = max(len(k) for k in kernels)
+    size = next((s for s in SUPPORTED_KERNEL_SIZES if longest <= s), None)
+    if size is None:
+        return False
+    padded = []
+    for k in kernels:
+        out = np.zeros(size, np.float32)
+        s0 = (size - len(k)) // 2
+        out[s0:s0 + len(k)] = np.asarray(k, np.float32)
+        padded.append(out)
+    w = int(dim[0])
+    h = int(dim[1]) if n > 1 else 1
+    d = int(dim[2]) if n > 2 else 1
+    kx = padded[0]
+    ky = padded[1] if n > 1 else None
+    kz = padded[2] if n > 2 else None
+    cuda = cuda or CUDASeparableConvolution()
+    return getattr(cuda, f"convolve_{size}")(img, kx, ky, kz, w, h, d, kx is not None, ky is not None,
+                                            kz is not None, oobs, oobs_value, device)
