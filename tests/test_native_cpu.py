@@ -28,6 +28,24 @@ def test_library_exports_every_header_symbol():
     assert sorted(declared) == sorted(_native.EXPORTS)
 
 
+def test_bindings_match_header_arity():
+    """Every ctypes declaration takes as many arguments as the header's prototype."""
+    txt = open(os.path.join(ROOT, "include", "mvdecon.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    L = _native.lib()
+    checked = 0
+    for name, params in re.findall(r"\b([A-Za-z_]\w*)\s*\(([^;{]*?)\)\s*;", txt, flags=re.S):
+        if name not in _native.EXPORTS:
+            continue
+        p = params.strip()
+        n = 0 if p in ("", "void") else p.count(",") + 1
+        argtypes = getattr(L, name).argtypes
+        if argtypes is not None or n == 0:
+            assert len(argtypes or []) == n, (name, n, argtypes)
+            checked += 1
+    assert checked >= 30
+
+
 def test_library_is_built_for_gfx950():
     data = open(_native.LIB_PATH, "rb").read()
     assert b"gfx950" in data
