@@ -4,7 +4,7 @@
 //  spim/process/fusion/weights/BlendingRealRandomAccess.java:44-121).
 //
 // Threads run along x (the image and weight stores of a wavefront are 256
-// contiguous bytes each), four voxels per thread.  The arithmetic
+// contiguous bytes each), four output rows per thread.  The arithmetic
 // keeps the reference's types and order so that results are bit-identical to
 // the CPU restatement: the inverse affine map in double, evaluated left to
 // right without fused multiply-adds and rounded to float; trilinear weights in
@@ -125,39 +125,38 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ raw, const 
   return acc;
 }
 
-// A workgroup covers kTfChunk voxels of one output row (z, y); each thread
-// kTfVox of them, kTfThreads apart (every store instruction of a wavefront
-// writes 256 contiguous bytes).  The row's terms of the affine map are
-// computed once per thread.
+// A workgroup covers kTfThreads consecutive x of kTfVox consecutive output rows
+// (same z); each thread walks its column down the rows, so the raw rows that
+// two neighbouring output rows share are fetched once into this CU's cache
+// (every store instruction of a wavefront still writes 256 contiguous bytes).
 constexpr int kTfVox = 4;
-constexpr int kTfChunk = kTfThreads * kTfVox;
 
 template <bool kOff32>
-__global__ __launch_bounds__(kTfThreads) void k_transform_view(TransformArgs a, int chunks) {
+__global__ __launch_bounds__(kTfThreads) void k_transform_view(TransformArgs a, int chunks, int ygroups) {
 #pragma clang fp contract(off)
-  const int row = blockIdx.x / chunks;  // z * ny + y
-  const int xb = (blockIdx.x - row * chunks) * kTfChunk + threadIdx.x;
-  const int y = row % a.ny, z = row / a.ny;
+  const int zy = blockIdx.x / chunks;  // z * ygroups + y group
+  const int x = (blockIdx.x - zy * chunks) * kTfThreads + threadIdx.x;
+  if (x >= a.nx) return;
+  const int z = zy / ygroups, y0 = (zy - z * ygroups) * kTfVox;
   // cursor position (float) + offset (int): TransformInput.java:79-83;
   // AffineTransform3D.applyInverse: (float)(m0*s0 + m1*s1 + m2*s2 + m3), left
-  // to right in double -- m1*s1 and m2*s2 are the row's terms
-  const double s1 = (double)((float)y + a.off[1]), s2 = (double)((float)z + a.off[2]);
-  double rb[3], rc[3];
+  // to right in double -- m0*s0 and m2*s2 are the column's terms
+  const double s0 = (double)((float)x + a.off[0]), s2 = (double)((float)z + a.off[2]);
+  double ra[3], rc[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    rb[r] = a.inv[4 * r + 1] * s1;
+    ra[r] = a.inv[4 * r] * s0;
     rc[r] = a.inv[4 * r + 2] * s2;
   }
-  const int64_t orow = (int64_t)row * a.nx;
 #pragma unroll 1
   for (int k = 0; k < kTfVox; ++k) {
-    const int x = xb + k * kTfThreads;
-    if (x >= a.nx) break;
-    const double s0 = (double)((float)x + a.off[0]);
+    const int y = y0 + k;
+    if (y >= a.ny) break;
+    const double s1 = (double)((float)y + a.off[1]);
     float t[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) t[r] = (float)(((a.inv[4 * r] * s0 + rb[r]) + rc[r]) + a.inv[4 * r + 3]);
-    const int64_t o = orow + x;
+    for (int r = 0; r < 3; ++r) t[r] = (float)(((ra[r] + a.inv[4 * r + 1] * s1) + rc[r]) + a.inv[4 * r + 3]);
+    const int64_t o = ((int64_t)z * a.ny + y) * a.nx + x;
     if (a.out_img) {
       float v = 0.f;  // outside the view: the reference's new image keeps 0
       // FusionHelper.intersects
@@ -192,7 +191,7 @@ const double* blend_lut_device(int device) {
 
 const char* check_transform_args(const TransformArgs& a) {
   if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0) return "output size must be positive";
-  if ((int64_t)a.ny * a.nz * ((a.nx + kTfChunk - 1) / kTfChunk) >= (int64_t(1) << 31))
+  if ((int64_t)((a.ny + kTfVox - 1) / kTfVox) * a.nz * ((a.nx + kTfThreads - 1) / kTfThreads) >= (int64_t(1) << 31))
     return "output too large for one launch";
   if (!a.out_img && !a.out_w) return "no output requested";
   if (a.rsize[0] <= 0 || a.rsize[1] <= 0 || a.rsize[2] <= 0) return "raw view size must be positive";
@@ -209,14 +208,17 @@ const char* check_transform_args(const TransformArgs& a) {
 }
 
 hipError_t launch_transform_view(const TransformArgs& a, hipStream_t stream) {
-  const int chunks = (a.nx + kTfChunk - 1) / kTfChunk;
-  const int64_t wgs = (int64_t)a.ny * a.nz * chunks;
+  const int chunks = (a.nx + kTfThreads - 1) / kTfThreads;
+  const int ygroups = (a.ny + kTfVox - 1) / kTfVox;
+  const int64_t wgs = (int64_t)ygroups * a.nz * chunks;
   const bool off32 = (int64_t)a.rsize[0] * a.rsize[1] * a.rsize[2] * 4 < (int64_t(1) << 32);
   (void)hipGetLastError();
   if (off32)
-    hipLaunchKernelGGL(k_transform_view<true>, dim3((unsigned)wgs), dim3(kTfThreads), 0, stream, a, chunks);
+    hipLaunchKernelGGL(k_transform_view<true>, dim3((unsigned)wgs), dim3(kTfThreads), 0, stream, a, chunks,
+                       ygroups);
   else
-    hipLaunchKernelGGL(k_transform_view<false>, dim3((unsigned)wgs), dim3(kTfThreads), 0, stream, a, chunks);
+    hipLaunchKernelGGL(k_transform_view<false>, dim3((unsigned)wgs), dim3(kTfThreads), 0, stream, a, chunks,
+                       ygroups);
   return hipGetLastError();
 }
 
