@@ -225,13 +225,30 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
   const auto out = gmem(P.spec) + (int64_t)z * kXYN * P.pitch;
   auto yaddr = [cl](int i) { return i * kXYLP + cl; };
   xy_fft<kXYN>(c0, lds, jy, tw, 1, -1.f, yaddr);
+  // Column 0 carries Z = F0 + i F128, F0 and F128 the transforms of the real
+  // DC and Nyquist columns: F0(k) = (Z(k) + conj Z(-k)) / 2 and
+  // F128(k) = (Z(k) - conj Z(-k)) / 2i.  Z goes through LDS so that the DC
+  // value is stored with columns 1..63 and the Nyquist value with columns
+  // 64..127 and the row padding (zeros): every 128-byte line of a row is
+  // written by one store instruction (a partly written line reaches HBM as a
+  // read-modify-write).
+  if (cl == 0) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) dcn[elem_index<kXYN, 16, SY::R1>(jy, 0, t)] = c0[t];
+  }
+  __syncthreads();
+  // the DC values of this wave's 16 rows, one per lane 0..15, handed to lane 0
+  // row by row
+  float2 sdc = make_float2(0.f, 0.f);
+  if (cl < 16) {
+    const int kk = elem_index<kXYN, 16, SY::R1>(jy, 0, cl);
+    sdc = cscale(cadd(dcn[kk], cconj(dcn[(kXYN - kk) & (kXYN - 1)])), 0.5f);
+  }
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    if (cl == 0)
-      dcn[ky] = c0[t];
-    else
-      out[ky * P.pitch + cl] = c0[t];
+    const float2 d0 = make_float2(__shfl(sdc.x, t), __shfl(sdc.y, t));
+    out[ky * P.pitch + cl] = cl == 0 ? d0 : c0[t];
   }
   xy_fft<kXYN>(c1, lds, jy, tw, 1, -1.f, yaddr);
 #pragma unroll
@@ -239,15 +256,18 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
     out[ky * P.pitch + 64 + cl] = c1[t];
   }
-  // separate the DC and Nyquist columns: Z = F0 + i F128 with F0 and F128 the
-  // transforms of real columns (the dcn writes are ordered by the barriers of
-  // the second round)
-  if (tid < kXYN) {
-    const int k = tid;
-    const float2 a = dcn[k], b = cconj(dcn[(kXYN - k) & (kXYN - 1)]);
-    const float2 s = cscale(cadd(a, b), 0.5f), d = cscale(csub(a, b), 0.5f);
-    out[k * P.pitch] = s;
-    out[k * P.pitch + kXYM] = make_float2(d.y, -d.x);
+  // the Nyquist column and the padding of this wave's 16 rows: 16 lanes per
+  // row, four rows per store instruction (pitch - 128 = 16 columns)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = 4 * q + (cl >> 4), c = cl & 15;
+    const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
+    float2 v = make_float2(0.f, 0.f);
+    if (c == 0) {
+      const float2 d = cscale(csub(dcn[ky], cconj(dcn[(kXYN - ky) & (kXYN - 1)])), 0.5f);
+      v = make_float2(d.y, -d.x);
+    }
+    out[ky * P.pitch + kXYM + c] = v;
   }
 }
 
@@ -502,7 +522,10 @@ const bool g_xy_disabled = [] {
 bool xy_gather_ok(int gather) { return gather == kGatherMirror || gather == kGatherOne || gather == kGatherZero; }
 }  // namespace
 
-bool xy_fused(const ShapePlan& sp) { return !g_xy_disabled && sp.q.x == kXYN && sp.q.y == kXYN; }
+// (the forward's padding stores assume rows of 144 = 128 + 16 complex values)
+bool xy_fused(const ShapePlan& sp) {
+  return !g_xy_disabled && sp.q.x == kXYN && sp.q.y == kXYN && sp.pitch == kXYM + 16;
+}
 
 hipError_t launch_xy_forward(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                              hipStream_t stream) {
