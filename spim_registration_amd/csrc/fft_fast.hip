@@ -28,8 +28,10 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
+  // lanes of the row padding [hx, pitch) transform zeros and store them, so
+  // that every 128-byte line of a row is written whole
   const int col = bx * L + line;
-  const bool valid = col < P.hx;
+  const bool valid = col < P.hx, store_col = col < P.pitch;
   const int z = P.zlo + by;
   const auto base = gmem(P.spec) + (int64_t)z * N * P.pitch + col;
   float2 v[E];
@@ -43,7 +45,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
   constexpr int RL = last_radix<N>();
-  if (valid) {
+  if (store_col) {
 #pragma unroll
     for (int b = 0; b < E / RL; ++b)
 #pragma unroll
@@ -272,6 +274,7 @@ struct XFwdFastParams {
 // run over whole rows so that the (y, z) bookkeeping is done once per row.
 template <int M, int MODE>
 __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int bx, int by) {
+  static_assert(M % 16 == 0, "the row pitch is M + 16");
   using S = Spec<M>;
   constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
   constexpr int W = 2 * M;                                   // real samples per row
@@ -359,7 +362,10 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
 #pragma unroll
     for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
   __syncthreads();
-  // unpack Z = FFT(even + i odd) into X[k] for the pair (k, M - k)
+  // unpack Z = FFT(even + i odd) into X[k] for the pair (k, M - k); X[M] (the
+  // pair of k = 0) goes out below with the row padding [M + 1, M + 16), zeros,
+  // so that the row's last 128-byte line is written by one store (a partly
+  // written line reaches HBM as a read-modify-write)
   constexpr int K2 = M / 2 + 1;
   for (int idx = threadIdx.x; idx < L * K2; idx += kThreads) {
     const int l = idx / K2, k = idx % K2;
@@ -372,11 +378,24 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
       const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
       out[k] = cadd(E2, cmul(tw[k], make_float2(D.y, -D.x)));
     }
-    if (M - k != k) {
+    if (M - k != k && k != 0) {
       const float2 ac = cconj(a);
       const float2 E2 = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
       out[M - k] = cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x)));
     }
+  }
+  for (int idx = threadIdx.x; idx < L * 16; idx += kThreads) {
+    const int l = idx >> 4, c = idx & 15;
+    const int row = row0 + l;
+    if (row >= nrows) continue;
+    float2 v = make_float2(0.f, 0.f);
+    if (c == 0) {
+      const float2 a = buf[l * MP];
+      const float2 ac = cconj(a);
+      const float2 E2 = cscale(cadd(a, ac), 0.5f), D = cscale(csub(a, ac), 0.5f);
+      v = cadd(E2, cmul(tw[M], make_float2(D.y, -D.x)));
+    }
+    gmem(P.spec)[(int64_t)row * P.pitch + M + c] = v;
   }
 }
 
