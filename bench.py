@@ -143,7 +143,6 @@ def run_c3(args):
         d.iterate(iters)
     torch.cuda.synchronize()
     res["plain_s"] = time.perf_counter() - t0
-    psi_plain = [d.get_psi() for d in plain]
     for d in plain:
         d.close()
     for name, policy in (("batched_same_shapes_s", FFTPolicy.SMOOTH), ("batched_s", FFTPolicy.AUTOTUNE)):
@@ -157,10 +156,6 @@ def run_c3(args):
         b.iterate(iters)
         torch.cuda.synchronize()
         res[name] = time.perf_counter() - t0
-        if policy == FFTPolicy.SMOOTH:
-            # same kernels and data as the plain path: identical estimates
-            # (one more iteration was run here, so compare after syncing counts)
-            pass
         b.close()
         for d in st:
             d.close()
@@ -490,7 +485,8 @@ def run_dog(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20 for the 512x512x256 workload, the workload's own count otherwise)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -502,6 +498,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="run the blocks of each view through the batch engine (one launch per pass, hipGraph)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"c3": 15, "c4": 5, "c5": 3, "resample": 10, "psf": 3, "dog": 3}.get(args.workload, 20)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)

@@ -22,6 +22,7 @@ struct FftPlan {
   int npass;
   int radix[kMaxPasses];
   int ns[kMaxPasses];    // product of the radices of the earlier passes
+  unsigned magic[kMaxPasses];  // ceil(2^32 / ns) for ns > 1: j / ns = umulhi(j, magic) for j * ns < 2^32
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -109,18 +110,29 @@ __device__ __forceinline__ void butterfly<8>(float2* v, float dir) {
 
 // One Stockham pass of radix R over all lines held in LDS.
 template <int R>
-__device__ __forceinline__ void fft_pass(float2* buf, int lp, int nlines, int n, int ns, const float2* tw,
-                                         int tw_step, float dir) {
+__device__ __forceinline__ void fft_pass(float2* buf, int lp, int nlines, int n, int ns, unsigned magic,
+                                         const float2* tw, int tw_step, float dir) {
   constexpr int kMaxB = (kLdsCap / R + kThreads - 1) / kThreads;
   const int m = n / R;
   const int total = m * nlines;
   const int tstride = (n / (ns * R)) * tw_step;
+  // index arithmetic without integer division: nlines is a power of two and
+  // j / ns comes from a multiply-high by the plan's reciprocal (j < 4096,
+  // ns <= n <= 1024, so j * ns < 2^32 and the quotient is exact)
+  const int lshift = __builtin_ctz((unsigned)nlines);
+  auto split = [&](int idx, int& line, int& j, int& k, int& q) {
+    line = idx & (nlines - 1);
+    j = idx >> lshift;
+    q = ns == 1 ? j : (int)__umulhi((unsigned)j, magic);
+    k = j - q * ns;
+  };
   float2 v[kMaxB][R];
 #pragma unroll
   for (int b = 0; b < kMaxB; ++b) {
     const int idx = threadIdx.x + b * kThreads;
     if (idx < total) {
-      const int line = idx % nlines, j = idx / nlines, k = j % ns;
+      int line, j, k, q;
+      split(idx, line, j, k, q);
 #pragma unroll
       for (int t = 0; t < R; ++t) {
         float2 x = buf[(j + t * m) * lp + line];
@@ -139,8 +151,9 @@ __device__ __forceinline__ void fft_pass(float2* buf, int lp, int nlines, int n,
   for (int b = 0; b < kMaxB; ++b) {
     const int idx = threadIdx.x + b * kThreads;
     if (idx < total) {
-      const int line = idx % nlines, j = idx / nlines, k = j % ns;
-      const int base = (j / ns) * ns * R + k;
+      int line, j, k, q;
+      split(idx, line, j, k, q);
+      const int base = q * ns * R + k;
 #pragma unroll
       for (int t = 0; t < R; ++t) buf[(base + t * ns) * lp + line] = v[b][t];
     }
@@ -148,18 +161,19 @@ __device__ __forceinline__ void fft_pass(float2* buf, int lp, int nlines, int n,
   __syncthreads();
 }
 
-// Full transform of the lines in LDS.  Caller synchronises before the call;
-// the function returns after a barrier.
+// Full transform of the lines in LDS (nlines a power of two).  Caller
+// synchronises before the call; the function returns after a barrier.
 __device__ __forceinline__ void lds_fft(float2* buf, int lp, int nlines, const FftPlan& plan, const float2* tw,
                                         int tw_step, float dir) {
   for (int p = 0; p < plan.npass; ++p) {
     const int r = plan.radix[p], ns = plan.ns[p];
+    const unsigned mg = plan.magic[p];
     switch (r) {
-      case 2: fft_pass<2>(buf, lp, nlines, plan.n, ns, tw, tw_step, dir); break;
-      case 3: fft_pass<3>(buf, lp, nlines, plan.n, ns, tw, tw_step, dir); break;
-      case 4: fft_pass<4>(buf, lp, nlines, plan.n, ns, tw, tw_step, dir); break;
-      case 5: fft_pass<5>(buf, lp, nlines, plan.n, ns, tw, tw_step, dir); break;
-      default: fft_pass<8>(buf, lp, nlines, plan.n, ns, tw, tw_step, dir); break;
+      case 2: fft_pass<2>(buf, lp, nlines, plan.n, ns, mg, tw, tw_step, dir); break;
+      case 3: fft_pass<3>(buf, lp, nlines, plan.n, ns, mg, tw, tw_step, dir); break;
+      case 4: fft_pass<4>(buf, lp, nlines, plan.n, ns, mg, tw, tw_step, dir); break;
+      case 5: fft_pass<5>(buf, lp, nlines, plan.n, ns, mg, tw, tw_step, dir); break;
+      default: fft_pass<8>(buf, lp, nlines, plan.n, ns, mg, tw, tw_step, dir); break;
     }
   }
 }

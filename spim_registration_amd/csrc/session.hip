@@ -64,6 +64,7 @@ FftPlan make_plan(int n) {
   auto push = [&](int r) {
     p.radix[p.npass] = r;
     p.ns[p.npass] = ns;
+    p.magic[p.npass] = ns > 1 ? (unsigned)((((uint64_t)1 << 32) + ns - 1) / ns) : 0u;
     ns *= r;
     ++p.npass;
     m /= r;
