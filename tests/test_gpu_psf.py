@@ -58,3 +58,14 @@ def test_extract_psf_container():
     assert e.get_transformed_psf(1).shape == tuple(reversed(P.transformed_psf_size((7, 7, 9), _model(90.0, 1.5))))
     avg = e.compute_average_psf()
     np.testing.assert_array_equal(avg, (e.original_psfs[0] + e.original_psfs[1]).astype(np.float32))
+
+
+def test_invalid_arguments_are_rejected():
+    from spim_registration_amd import _native
+    img = np.ones((10, 10, 10), np.float32)
+    with pytest.raises(_native.NativeError):
+        P.extract_psf(img, [(5.0, 5.0, 5.0)], (0, 5, 5))
+    m = _model(0.0, 1.0).copy()
+    m[0] = np.inf
+    with pytest.raises(_native.NativeError):
+        P.transformed_psf_size((5, 5, 5), m)

@@ -143,3 +143,27 @@ def test_device_entry_matches_host_entry():
         torch.cuda.synchronize()
         np.testing.assert_array_equal(oi.cpu().numpy(), img)
         np.testing.assert_array_equal(ow.cpu().numpy(), w)
+
+
+def test_invalid_arguments_are_rejected():
+    """Argument errors come back as NativeError before any launch."""
+    from spim_registration_amd import _native
+    raw = np.ones((4, 5, 6), np.float32)
+    with pytest.raises(_native.NativeError):
+        T.transform_view(raw, IDENT, (0, 0, 0), (0, 5, 4))  # empty box
+    bad = IDENT.copy()
+    bad[5] = np.nan
+    with pytest.raises(ValueError):
+        T.transform_view(raw, bad, (0, 0, 0), (6, 5, 4))  # caught by the wrapper
+    import ctypes
+    L = _native.lib()
+    inv = np.ascontiguousarray(bad)
+    off = np.zeros(3, np.int64)
+    out = np.empty((4, 5, 6), np.float32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert L.mvd_transform_view(0, p(raw), 6, 5, 4, p(inv), p(off), 6, 5, 4, None, None, p(out), None) != 0
+    b = np.zeros(3, np.float32)
+    r = np.zeros(3, np.float32)  # range 0 is invalid
+    w = np.empty_like(out)
+    assert L.mvd_transform_view(0, p(raw), 6, 5, 4, p(IDENT), p(off), 6, 5, 4, p(b), p(r), None, p(w)) != 0
+    assert L.mvd_transform_view(99, p(raw), 6, 5, 4, p(IDENT), p(off), 6, 5, 4, None, None, p(out), None) != 0
