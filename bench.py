@@ -501,7 +501,7 @@ def main():
     if args.steps is None:
         args.steps = {"c3": 15, "c4": 5, "c5": 3, "resample": 10, "psf": 3, "dog": 3}.get(args.workload, 20)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not supported",
               file=sys.stderr)
         sys.exit(2)
 
