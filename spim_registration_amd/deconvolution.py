@@ -251,7 +251,13 @@ class MVDeconvolution:
         for v in vs:
             self._s.add(v)
         if weight_type is None:
-            weight_type = WeightType.AS_GIVEN if all(v.has_weight() for v in vs) else WeightType.NO_WEIGHTS
+            # blending weights resampled on the GPU are raw (ProcessForDeconvolution
+            # normalises them: PRECOMPUTED); weights handed in are taken as given
+            if all(v.has_weight() for v in vs):
+                raw_blend = any(v._raw is not None for v in vs)
+                weight_type = WeightType.PRECOMPUTED_WEIGHTS if raw_blend else WeightType.AS_GIVEN
+            else:
+                weight_type = WeightType.NO_WEIGHTS
         self.weight_type = WeightType(weight_type)
         mm = (ctypes.c_double * 2)()
         L = _native.lib()
