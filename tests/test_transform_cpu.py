@@ -165,3 +165,19 @@ def test_inverse_affine_and_bounding_box():
     np.testing.assert_allclose(inv[:, :3] @ q + inv[:, 3], p, atol=1e-12)
     mn, size = T.bounding_box([(10, 10, 10)], [np.hstack([np.eye(3), [[2], [-3], [0.5]]])])
     assert mn == (2, -3, 0) and size == (10, 10, 11)
+
+
+def test_raw_view_host_logic():
+    """MVDeconFFT.from_raw keeps the raw view and reports the box shape; the
+    deconvolution defaults to PRECOMPUTED weights for blended raw views."""
+    from spim_registration_amd import MVDeconFFT
+    raw = _raw((6, 7, 8), 9)
+    psf = np.ones((3, 3, 3), np.float32)
+    v = MVDeconFFT.from_raw(raw, IDENT, (0, 0, 0), (10, 9, 8), psf, T.Blending())
+    assert v.shape == (8, 9, 10) and v.has_weight()
+    w = MVDeconFFT.from_raw(raw, IDENT, (0, 0, 0), (10, 9, 8), psf)
+    assert not w.has_weight()
+    with pytest.raises(ValueError):
+        MVDeconFFT.from_raw(raw, IDENT[:6], (0, 0, 0), (10, 9, 8), psf)
+    with pytest.raises(ValueError):
+        T.Blending((1.0, 2.0, 3.0), (0.0, 1.0, 1.0)).arrays()
