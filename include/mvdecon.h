@@ -235,6 +235,19 @@ int mvd_get_kernels(mvd_session* s, int view, float* kernel1, float* kernel2);
  * re-run its init (normalisation, kernel2, spectra). */
 int mvd_set_psf(mvd_session* s, int view, const float* psf);
 
+/* setKernel for every view at once (MVDeconFFT.java:364-369 applied to all
+ * views, then each view's init once): psfs[v] is view v's new PSF.  The state
+ * equals calling mvd_set_psf for all views twice in a row -- the compound
+ * kernels of a view read the other views' kernel1, so a single pass of
+ * setKernel leaves the earlier views prepared from older PSFs -- at half the
+ * preparation work. */
+int mvd_set_psfs(mvd_session* s, const float* const* psfs);
+
+/* Sum of the current (unmasked) estimate in double precision (no counterpart
+ * in the reference: the total S of the PSF refinement step, computed where
+ * the estimate lives instead of after a copy to the host). */
+int mvd_psi_sum(mvd_session* s, double* out);
+
 /* Poisson likelihood of the multiview forward model and its gradient (no
  * counterpart in the reference; its forward model is convolve1,
  * MVDeconFFT.java:384-470):  b_v = E_mirror(psi) * P_v,

@@ -23,7 +23,7 @@ EXPORTS = (
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_add_view_transformed", "mvd_transform_view",
     "mvd_transform_view_device", "mvd_extract_psf", "mvd_psf_transform_size",
     "mvd_set_weights", "mvd_prepare", "mvd_init_psi",
-    "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_timing", "mvd_get_timing",
+    "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_psfs", "mvd_psi_sum", "mvd_set_timing", "mvd_get_timing",
     "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_nll_grad_device",
     "mvd_get_grad_psi", "mvd_set_fft_policy",
     "mvd_batch_create", "mvd_batch_add", "mvd_batch_prepare", "mvd_batch_iterate", "mvd_batch_destroy",
@@ -75,6 +75,8 @@ def _declare(L):
     L.mvd_get_psi.argtypes = [_P, _P, ctypes.c_int]
     L.mvd_get_kernels.argtypes = [_P, ctypes.c_int, _P, _P]
     L.mvd_set_psf.argtypes = [_P, ctypes.c_int, _P]
+    L.mvd_set_psfs.argtypes = [_P, _P]
+    L.mvd_psi_sum.argtypes = [_P, _P]
     L.mvd_set_timing.argtypes = [_P, ctypes.c_int]
     L.mvd_get_timing.argtypes = [_P, _P, _P, ctypes.c_int]
     L.mvd_timing_name.argtypes = [ctypes.c_int]

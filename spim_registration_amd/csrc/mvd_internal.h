@@ -194,6 +194,7 @@ hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, in
                                hipStream_t stream);
 hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
                                   unsigned long long* part_cnt, int nparts, hipStream_t stream);
+hipError_t launch_sum(const float* a, int64_t n, double* part_sum, int nparts, hipStream_t stream);
 hipError_t launch_fill(float* dst, float v, int64_t n, hipStream_t stream);
 hipError_t launch_mask(const float* psi, const float* counts, float* out, int64_t n, hipStream_t stream);
 hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_t stride, float* wsum,

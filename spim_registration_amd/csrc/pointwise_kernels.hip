@@ -62,6 +62,22 @@ hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, flo
   return hipGetLastError();
 }
 
+// Sum of a float array in double, one partial per workgroup.
+__global__ __launch_bounds__(kPwThreads) void k_sum(const float* a, int64_t n, double* part_sum) {
+  __shared__ double rs[kPwThreads / 64];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kPwThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPwThreads)
+    s += (double)a[i];
+  const double bs = block_sum(s, rs);
+  if (threadIdx.x == 0) part_sum[blockIdx.x] = bs;
+}
+
+hipError_t launch_sum(const float* a, int64_t n, double* part_sum, int nparts, hipStream_t stream) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_sum, dim3(nparts), dim3(kPwThreads), 0, stream, a, n, part_sum);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kPwThreads) void k_fill(float* dst, float v, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)kPwThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPwThreads)
     dst[i] = v;

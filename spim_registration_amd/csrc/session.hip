@@ -773,11 +773,12 @@ int same_conv_zero(mvd_session* s, const std::vector<float>& in, Int3 din, const
   return 0;
 }
 
-// MVDeconFFT.init for one view (deconv/MVDeconFFT.java:183-324).
-int init_view(mvd_session* s, int v) {
+// MVDeconFFT.init for one view (deconv/MVDeconFFT.java:183-324).  With
+// normalise false, kernel1 is taken as already normalised.
+int init_view(mvd_session* s, int v, bool normalise = true) {
   auto& V = s->views;
   mvd_session::View& view = V[v];
-  norm_img(view.k1, s->norm_threads);
+  if (normalise) norm_img(view.k1, s->norm_threads);
   const int nv = s->nviews;
   const int64_t kn = (int64_t)view.k1.size();
   if (nv == 1 || s->psf_type == MVD_PSF_INDEPENDENT) {
@@ -1292,6 +1293,44 @@ int mvd_set_psf(mvd_session* s, int view, const float* psf) {
   int rc = init_view(s, view);
   if (rc) return rc;
   return view_spectra(s, view);
+}
+
+int mvd_set_psfs(mvd_session* s, const float* const* psfs) {
+  MVD_REQUIRE(s && psfs, "NULL argument");
+  MVD_REQUIRE(s->prepared, "mvd_prepare first");
+  for (int v = 0; v < s->nviews; ++v) MVD_REQUIRE(psfs[v], "NULL PSF");
+  DeviceGuard guard(s->device);
+  // every kernel1 first: the compound kernels of each view read the others'
+  for (int v = 0; v < s->nviews; ++v) {
+    auto& view = s->views[v];
+    view.psf_raw.assign(psfs[v], psfs[v] + view.k1.size());
+    view.k1 = view.psf_raw;
+    norm_img(view.k1, s->norm_threads);
+  }
+  s->bb.built = false;
+  for (int v = 0; v < s->nviews; ++v) {
+    int rc = init_view(s, v, false);
+    if (rc) return rc;
+    rc = view_spectra(s, v);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int mvd_psi_sum(mvd_session* s, double* out) {
+  MVD_REQUIRE(s && out, "NULL argument");
+  MVD_REQUIRE(s->psi_ready, "mvd_init_psi first");
+  DeviceGuard guard(s->device);
+  MVD_CHECK(s->d_fi_sum.ensure(sizeof(double) * kReduceParts));
+  const int parts = (int)std::min<int64_t>(kReduceParts, std::max<int64_t>(1, (s->n + 255) / 256));
+  MVD_CHECK(launch_sum(s->d_psi.as<float>(), s->n, s->d_fi_sum.as<double>(), parts, s->stream));
+  std::vector<double> ps(parts);
+  MVD_CHECK(hipMemcpyAsync(ps.data(), s->d_fi_sum.p, sizeof(double) * parts, hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(hipStreamSynchronize(s->stream));
+  double t = 0;
+  for (double v : ps) t += v;
+  *out = t;
+  return 0;
 }
 
 int mvd_init_psi(mvd_session* s, const float* psi0, double* avg_out) {

@@ -346,6 +346,28 @@ class MVDeconvolution:
         _native.check(_native.lib().mvd_get_grad_psi(self._s.handle, _ptr(out)), "mvd_get_grad_psi")
         return out
 
+    def psi_sum(self) -> float:
+        """Sum of the unmasked estimate in double precision, on the device
+        (mvd_psi_sum)."""
+        out = ctypes.c_double()
+        _native.check(_native.lib().mvd_psi_sum(self._s.handle, ctypes.byref(out)), "mvd_psi_sum")
+        return out.value
+
+    def set_kernels(self, kernels):
+        """setKernel for every view at once (mvd_set_psfs): the same kernels
+        as calling MVDeconFFT.setKernel (:364-369) for all views twice, each
+        view prepared once from the new PSFs of all views."""
+        views = self._s.views
+        if len(kernels) != len(views):
+            raise ValueError("one kernel per view")
+        ks = [_f32(k) for k in kernels]
+        for k, v in zip(ks, views):
+            if k.shape != v.kernel1.shape:
+                raise ValueError("kernel size cannot change")
+        arr = (ctypes.c_void_p * len(ks))(*[k.ctypes.data for k in ks])
+        _native.check(_native.lib().mvd_set_psfs(self._s.handle, arr), "mvd_set_psfs")
+        self._s._fetch_kernels()
+
     def set_timing(self, on: bool):
         _native.check(_native.lib().mvd_set_timing(self._s.handle, 1 if on else 0), "mvd_set_timing")
 

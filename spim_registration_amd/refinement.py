@@ -60,27 +60,25 @@ def refine_psfs(decon: MVDeconvolution, steps: int, iterations: int = 1, eta: fl
     history: List[RefinementStep] = []
     for step in range(steps):
         decon.iterate(iterations)
-        psi = decon.get_psi(masked=False)
+        if weights is None:
+            # S_v: the total estimate, the same for every view
+            totals = [decon.psi_sum()] * len(views)
+        else:
+            psi = decon.get_psi(masked=False).astype(np.float64)
+            totals = [float(psi.sum() if w is None else (psi * w).sum()) for w in weights]
         # the gradient at the kernels the iteration uses (kernel1: the given PSF
         # after normImg), not at the PSFs as handed in
         kernels = [view.get_kernel1() for view in views]
         nll, _, gpsf = decon.nll_grad(psfs=kernels, want_psi=False, want_psf=True)
         change = 0.0
         new_psfs = []
-        for v, view in enumerate(views):
-            w = None if weights is None else weights[v]
-            total = float(psi.astype(np.float64).sum() if w is None else (psi.astype(np.float64) * w).sum())
+        for v in range(len(views)):
             old = kernels[v]
-            new = em_psf_update(old, gpsf[v], total, eta)
+            new = em_psf_update(old, gpsf[v], totals[v], eta)
             change = max(change, float(np.abs(new - old).max() / max(np.abs(old).max(), 1e-30)))
             new_psfs.append(new)
-        # setKernel re-prepares one view from the PSFs known at that moment; the
-        # compound kernels couple the views, so every view is prepared again
-        # once all PSFs are new
-        for view, new in zip(views, new_psfs):
-            view.set_kernel(new)
-        if len(views) > 1:
-            for view, new in zip(views, new_psfs):
-                view.set_kernel(new)
+        # every view prepared once from the new PSFs of all views (the compound
+        # kernels couple the views)
+        decon.set_kernels(new_psfs)
         history.append(RefinementStep(step, nll, change))
     return history

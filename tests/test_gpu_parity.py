@@ -212,6 +212,35 @@ def test_set_kernel_reinitialises_view():
     d.close()
 
 
+@pytest.mark.parametrize("ptype", [PSFTYPE.EFFICIENT_BAYESIAN, PSFTYPE.OPTIMIZATION_I])
+def test_set_kernels_equals_set_kernel_twice(ptype):
+    """mvd_set_psfs leaves the state of setKernel over all views, twice: the
+    same kernels bit for bit, the oracle's kernels of the new PSFs, and the
+    same iterations afterwards."""
+    ds = _small_dataset(seed=6)
+    newks = [S.psf((7, 7, 9), 1.0 + 0.1 * v, 2.0, 45 + 50 * v) for v in range(3)]
+    a = _session(ds.views, ds.psfs, ptype, 1, 0.0)
+    b = _session(ds.views, ds.psfs, ptype, 1, 0.0)
+    a.set_kernels(newks)
+    for _ in range(2):
+        for v, k in enumerate(newks):
+            b.get_data().get_views()[v].set_kernel(k)
+    k1, k2 = O.init_kernels(newks, int(ptype), 8)
+    for v in range(3):
+        va, vb = a.get_data().get_views()[v], b.get_data().get_views()[v]
+        np.testing.assert_array_equal(va.get_kernel1(), vb.get_kernel1())
+        np.testing.assert_array_equal(va.get_kernel2(), vb.get_kernel2())
+        np.testing.assert_array_equal(va.get_kernel1(), k1[v])
+        assert rel_max(va.get_kernel2(), k2[v]) < 2e-5
+    a.iterate(2)
+    b.iterate(2)
+    np.testing.assert_array_equal(a.get_psi(), b.get_psi())
+    with pytest.raises(ValueError):
+        a.set_kernels(newks[:2])
+    a.close()
+    b.close()
+
+
 def test_config1_parity():
     """Config 1: 3 views of 128x128x64, PSF 19x19x31, 10 iterations, lambda 0,
     whole image as one block; against the float32 oracle (gate) and the FP64

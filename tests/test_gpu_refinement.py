@@ -43,3 +43,26 @@ def test_refinement_trajectory_matches_oracle():
         print(f"PSF after 3 steps: max rel err {err:.2e}", flush=True)
         assert err < 1e-4
     d.close()
+
+
+def test_psi_sum_and_weighted_totals():
+    """mvd_psi_sum equals the double sum of the fetched estimate; unit weights
+    in refine_psfs (host totals) give the trajectory of no weights (device
+    total)."""
+    k = (7, 7, 9)
+    ds = S.make_dataset((28, 24, 20), 2, [0, 90], 20, k, 1.2, 3.0, seed=22)
+    runs = []
+    for weights in (None, [np.ones((20, 24, 28), np.float32), None]):
+        inp = MVDeconInput()
+        for v, p in zip(ds.views, ds.psfs):
+            inp.add(MVDeconFFT(v, None, p, [0]))
+        d = MVDeconvolution(inp, PSFTYPE.EFFICIENT_BAYESIAN, 2, 0.0)
+        tot = d.psi_sum()
+        ref = d.get_psi(masked=False).astype(np.float64).sum()
+        assert abs(tot - ref) <= 1e-12 * abs(ref)
+        hist = refine_psfs(d, 2, weights=weights)
+        runs.append(([v.get_kernel1() for v in d.get_data().get_views()], [h.nll for h in hist]))
+        d.close()
+    for a, b in zip(runs[0][0], runs[1][0]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-6 * np.abs(b).max())
+    np.testing.assert_allclose(runs[0][1], runs[1][1], rtol=1e-12)
