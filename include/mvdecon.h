@@ -300,6 +300,49 @@ int mvd_batch_prepare(mvd_batch* b, int use_graph);
 int mvd_batch_iterate(mvd_batch* b, int n, double* stats);
 int mvd_batch_destroy(mvd_batch* b);
 
+/* ---- Output path of the deconvolved image ---------------------------------
+ * EfficientBayesianBased.java:294-299 hands the result to an ImgExport with
+ * the display range (0, 1); the exporters compute a missing range with
+ * FusionHelper.minMax and the HDF5 export converts to 16 bit.  A TIFF file is
+ * the counterpart of Save3dTIFF.exportImage (Save3dTIFF.java:79-133, ImageJ
+ * FileSaver.saveAsTiffStack): one uncompressed page per z slice, an ImageJ
+ * description (images, slices, min, max, spacing and origin when calibrated),
+ * big-endian unless big_endian == 0; BigTIFF when the file needs 64-bit
+ * offsets or bigtiff != 0.  Device inputs must be complete (synchronised)
+ * when passed. */
+
+/* FusionHelper.minMax (FusionHelper.java:108-172): minmax[0..1] = min, max
+ * with java.lang.Math.min/max semantics (NaN propagates, -0.0 < 0.0), from
+ * +/-Float.MAX_VALUE.  img: n floats in host memory (copied) or, for the
+ * _device form, in device memory of `device` (stream: hipStream_t or NULL). */
+int mvd_min_max(int device, const float* img, long long n, float* minmax);
+int mvd_min_max_device(int device, const float* img, long long n, float* minmax, void* stream);
+
+/* ExportSpimData2HDF5.updateAndGetMinMax (ExportSpimData2HDF5.java:227-253):
+ * out = {max(0, min - (min + max) / 2 * 0.1), max + (min' + max) / 2 * 0.1}
+ * with min' the widened min. */
+int mvd_u16_range(const float* minmax, double* out);
+
+/* RealUnsignedShortConverter(min, max) (imglib2 6.1.0, used at
+ * ExportSpimData2HDF5.java:256-262) on n floats in device memory: out[i] =
+ * min(65535, (int)(max(0, (img[i] - min) / (max - min) * 65535) + 0.5)),
+ * NaN -> 0; byte-swapped when big_endian != 0.  Asynchronous on stream. */
+int mvd_convert_u16_device(int device, const float* img, long long n, double min, double max, int big_endian,
+                           unsigned short* out, void* stream);
+
+/* Write a [nz][ny][nx] float image as a TIFF stack.  bits 32: float pages,
+ * display range (min, max), NaN -> minMax; bits 16: the 16-bit conversion
+ * over (min, max), NaN -> the widened minMax.  calibration (nullable) =
+ * {xorigin, yorigin, zorigin, pixel size} as Save3dTIFF.java:103-108 sets it
+ * from the bounding box.  img_on_device: img is device memory of `device`. */
+int mvd_write_tiff(int device, const float* img, int img_on_device, int nx, int ny, int nz, int bits, double min,
+                   double max, const double* calibration, int big_endian, int bigtiff, const char* path);
+
+/* exporter.exportImage(getPsi(), ...) of a session: the masked estimate
+ * written from device memory as by mvd_write_tiff. */
+int mvd_export_psi(mvd_session* s, const char* path, int bits, double min, double max, const double* calibration,
+                   int big_endian, int bigtiff);
+
 /* Thread-local message of the last failed call. */
 const char* mvd_last_error(void);
 

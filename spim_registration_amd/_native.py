@@ -26,6 +26,8 @@ EXPORTS = (
     "mvd_iterate", "mvd_get_psi", "mvd_get_kernels", "mvd_set_psf", "mvd_set_psfs", "mvd_psi_sum", "mvd_set_timing", "mvd_get_timing",
     "mvd_timing_name", "mvd_get_geometry", "mvd_last_error", "mvd_nll_grad", "mvd_nll_grad_device",
     "mvd_get_grad_psi", "mvd_set_fft_policy",
+    "mvd_min_max", "mvd_min_max_device", "mvd_u16_range", "mvd_convert_u16_device", "mvd_write_tiff",
+    "mvd_export_psi",
     "mvd_batch_create", "mvd_batch_add", "mvd_batch_prepare", "mvd_batch_iterate", "mvd_batch_destroy",
 )
 
@@ -77,6 +79,16 @@ def _declare(L):
     L.mvd_set_psf.argtypes = [_P, ctypes.c_int, _P]
     L.mvd_set_psfs.argtypes = [_P, _P]
     L.mvd_psi_sum.argtypes = [_P, _P]
+    L.mvd_min_max.argtypes = [ctypes.c_int, _P, ctypes.c_longlong, _P]
+    L.mvd_min_max_device.argtypes = [ctypes.c_int, _P, ctypes.c_longlong, _P, _P]
+    L.mvd_u16_range.argtypes = [_P, _P]
+    L.mvd_convert_u16_device.argtypes = [ctypes.c_int, _P, ctypes.c_longlong, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, _P, _P]
+    L.mvd_write_tiff.argtypes = [ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_double, _P, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_char_p]
+    L.mvd_export_psi.argtypes = [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P,
+                                 ctypes.c_int, ctypes.c_int]
     L.mvd_set_timing.argtypes = [_P, ctypes.c_int]
     L.mvd_get_timing.argtypes = [_P, _P, _P, ctypes.c_int]
     L.mvd_timing_name.argtypes = [ctypes.c_int]

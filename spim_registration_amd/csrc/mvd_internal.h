@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -195,6 +196,12 @@ hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, in
 hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
                                   unsigned long long* part_cnt, int nparts, hipStream_t stream);
 hipError_t launch_sum(const float* a, int64_t n, double* part_sum, int nparts, hipStream_t stream);
+// Output path (export.hip): min/max with Math.min/max semantics, the 16-bit
+// range widening and the TIFF stack writer (img in device or host memory).
+hipError_t launch_min_max(const float* img, int64_t n, float* minmax, hipStream_t stream);
+void u16_range(const float* minmax, double* out);
+int write_tiff_stack(const float* img, bool on_device, int nx, int ny, int nz, int bits, double min, double max,
+                     const double* calib, bool big_endian, int force_bigtiff, const char* path, hipStream_t stream);
 hipError_t launch_fill(float* dst, float v, int64_t n, hipStream_t stream);
 hipError_t launch_mask(const float* psi, const float* counts, float* out, int64_t n, hipStream_t stream);
 hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_t stride, float* wsum,

@@ -1453,6 +1453,22 @@ int mvd_get_psi(mvd_session* s, float* out, int masked) {
   return 0;
 }
 
+int mvd_export_psi(mvd_session* s, const char* path, int bits, double min, double max, const double* calibration,
+                   int big_endian, int bigtiff) {
+  MVD_REQUIRE(s && path, "NULL argument");
+  MVD_REQUIRE(s->psi_ready, "mvd_init_psi first");
+  DeviceGuard guard(s->device);
+  // the masked estimate (getPsi) into the ratio buffer: view counts first,
+  // then psi where a view has data, in place
+  double sum;
+  unsigned long long cnt;
+  int rc = first_iteration_sums(s, s->d_ratio.as<float>(), &sum, &cnt);
+  if (rc) return rc;
+  MVD_CHECK(launch_mask(s->d_psi.as<float>(), s->d_ratio.as<float>(), s->d_ratio.as<float>(), s->n, s->stream));
+  return write_tiff_stack(s->d_ratio.as<float>(), true, s->img.x, s->img.y, s->img.z, bits, min, max, calibration,
+                          big_endian != 0, bigtiff, path, s->stream);
+}
+
 int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* psfs, double* nll, int want_grad_psi,
                         float* const* grad_psf) {
   MVD_REQUIRE(s, "NULL session");

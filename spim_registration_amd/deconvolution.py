@@ -368,6 +368,19 @@ class MVDeconvolution:
         _native.check(_native.lib().mvd_set_psfs(self._s.handle, arr), "mvd_set_psfs")
         self._s._fetch_kernels()
 
+    def export_tiff(self, path: str, bits: int = 32, min: float = 0.0, max: float = 1.0, calibration=None,
+                    big_endian: bool = True, bigtiff: bool = False) -> str:
+        """exporter.exportImage(getPsi(), bb, tp, vs, 0, 1)
+        (EfficientBayesianBased.java:294-299) as a TIFF stack written from
+        device memory (mvd_export_psi); NaN min/max -> minMax of the image."""
+        import os
+        cal = None if calibration is None else np.ascontiguousarray(calibration, np.float64).reshape(4)
+        _native.check(_native.lib().mvd_export_psi(
+            self._s.handle, os.fsencode(path), int(bits), float(min), float(max),
+            None if cal is None else cal.ctypes.data_as(ctypes.c_void_p), 1 if big_endian else 0,
+            1 if bigtiff else 0), "mvd_export_psi")
+        return path
+
     def set_timing(self, on: bool):
         _native.check(_native.lib().mvd_set_timing(self._s.handle, 1 if on else 0), "mvd_set_timing")
 
