@@ -582,6 +582,11 @@ def run_export(args):
         csec = time.perf_counter() - t0
         cpu = dict(value=round(a.size / csec / 1e9, 4), unit="Gvoxel/s", cores=1, kind="port",
                    sample=f"numpy oracle min/max + 16-bit conversion of 32 of the 512 planes, {csec:.2f} s")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_export.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("k_to_u16")
     res = {
         "metric": "output path: min/max + 16-bit conversion of the estimate, voxels per second (inputs in HBM)",
         "value": round(value, 3), "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -589,7 +594,7 @@ def run_export(args):
         "dtype": "f32 -> u16 (f64 arithmetic)", "data": "synthetic",
         "config": {"workload": "export: 1024x1024x512 f32 estimate, minMax, widened range, 16-bit big-endian"},
         "roofline": {"bound": "hbm", "kernel": "k_to_u16", "achieved": round(conv_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(conv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_us": round(conv_us, 2), "model": "4 bytes read + 2 bytes written per voxel"},
         "min_max": {"avg_us": round(mm_us, 2), "GB_s": round(mm_gbs, 1), "model": "4 bytes read per voxel"},
         "parity": {"min_max_bit_exact": exact_mm, "u16_planes_100_107_bit_exact": exact},
