@@ -29,6 +29,28 @@ hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, in
   return hipGetLastError();
 }
 
+// out (+)= conj(a) b: the PSF correlation spectra of the blocks summed in the
+// spectral domain (the correlation is linear and every block's window frame
+// gives the same lags), written on the first block, added afterwards.
+__global__ __launch_bounds__(kGThreads) void k_spec_corr_acc(float2* out, const float2* a, const float2* b, int first,
+                                                            int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads) {
+    float2 x = a[i];
+    x.y = -x.y;
+    const float2 p = cmul(x, b[i]);
+    out[i] = first ? p : cadd(out[i], p);
+  }
+}
+
+hipError_t launch_spec_corr_acc(float2* out, const float2* a, const float2* b, int first, int64_t n,
+                                hipStream_t stream) {
+  const int64_t blocks = std::min<int64_t>((n + kGThreads - 1) / kGThreads, 16384);
+  if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_spec_corr_acc, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, out, a, b, first, n);
+  return hipGetLastError();
+}
+
 // grad[j] (+)= scale * box[(c - j) mod q]: the correlation corr(R, W)(s)
 // evaluated at s = c - j for every kernel voxel j; accumulate adds the block's
 // share to the view's gradient.

@@ -273,6 +273,9 @@ hipError_t launch_reduce_stats_batch(const ReduceItem* items, int nitems, hipStr
 // out = op(a) * op(b) over n spectrum values, op = conjugate when the flag is set.
 hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, int conj_a, int conj_b, int64_t n,
                                hipStream_t stream);
+// out = conj(a) b when first, else out += conj(a) b.
+hipError_t launch_spec_corr_acc(float2* out, const float2* a, const float2* b, int first, int64_t n,
+                                hipStream_t stream);
 // grad[j] = scale * box[(c - j) mod q] for the kernel box kd (x-fastest).
 hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, int accumulate,
                                    hipStream_t stream);

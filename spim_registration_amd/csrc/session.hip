@@ -423,7 +423,7 @@ struct mvd_session {
   DeviceBuf d_views, d_weights, d_wsum, d_psi, d_ratio, d_spec, d_part_sum, d_part_max, d_stats, d_tmp;
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
-  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
+  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
   DeviceBuf g_crop, g_nll, g_gpsi;
   bool g_gpsi_ready = false;
   PlanCache plans;
@@ -1527,6 +1527,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_spec_a.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_b.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_c.ensure(sizeof(float2) * hs));
+    if (grad_psf && grad_psf[v] && blocks.size() > 1) MVD_CHECK(s->g_spec_d.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_box.ensure(sizeof(float) * box));
     MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
     MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
@@ -1574,21 +1575,25 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       MVD_CHECK(launch_xy_forward_passes(sp, gc, s->g_crop.as<float>(), kGatherZero, Cs, st));
       MVD_CHECK(launch_z(sp, Cs, nullptr, 1.f, 0, q.z, st, -1.f));
       if (want_psf) {
-        // dL/dP(j) += corr(R, W)(c - j) = IFFT(conj(R^) W^)(c - j) / (qx qy qz)
-        MVD_CHECK(launch_spec_product(B, Cs, A, 1, 0, hs, st));
-        MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, q.z, st, 1.f));
-        ConvGeom gb{};
-        gb.src = q;
-        gb.win = q;
-        gb.q = q;
-        gb.eff_size = q;
-        gb.dst = q;
-        EpilogueArgs es{};
-        es.mode = kEpiStore;
-        es.dst = s->g_box.as<float>();
-        MVD_CHECK(launch_yx_inverse_passes(sp, gb, B, es, st, nullptr));
-        const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
-        MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, s->g_kgrad.as<float>(), bi > 0, st));
+        // dL/dP(j) = sum over blocks of corr(R, W)(c - j) = IFFT(sum conj(R^) W^)(c - j) / (qx qy qz):
+        // the products are summed here and inverted once after the last block
+        float2* D = blocks.size() > 1 ? s->g_spec_d.as<float2>() : B;
+        MVD_CHECK(launch_spec_corr_acc(D, Cs, A, bi == 0, hs, st));
+        if (bi + 1 == blocks.size()) {
+          MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));
+          ConvGeom gb{};
+          gb.src = q;
+          gb.win = q;
+          gb.q = q;
+          gb.eff_size = q;
+          gb.dst = q;
+          EpilogueArgs es{};
+          es.mode = kEpiStore;
+          es.dst = s->g_box.as<float>();
+          MVD_CHECK(launch_yx_inverse_passes(sp, gb, D, es, st, nullptr));
+          const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
+          MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, s->g_kgrad.as<float>(), 0, st));
+        }
       }
       if (want_grad_psi) {
         // adjoint on the block window: IFFT(R^ conj(P^)), folded back through the mirror extension
