@@ -98,24 +98,85 @@ hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 s
 }
 
 // Adjoint of the mirror-single window gather: every window position u in
-// [lo, lo + wd) adds its value to the image voxel mirror(u).  Positions of one
-// window can share a voxel near the borders, hence the FP64 atomic adds
-// (accumulation order, and with it the last bits, is not fixed).
-__global__ __launch_bounds__(kGThreads) void k_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc) {
-  const int64_t total = (int64_t)wd.x * wd.y * wd.z;
+// [lo, lo + wd) adds its value to the image voxel mirror(u).  Positions inside
+// the image map one to one onto image voxels: they are added with plain
+// read-add-writes over the direct box [da, db) (window coordinates).  The
+// positions outside the image on some axis (the mirrored shell of a window
+// that crosses an image face) can share voxels with each other and with the
+// direct box: a second launch, after the first on the same stream, adds them
+// with FP64 atomics (their order, and with it the last bits, is not fixed).
+__global__ __launch_bounds__(kGThreads) void k_fold_direct(const float* win, Int3 wd, Int3 lo, Int3 n, Int3 da,
+                                                          Int3 db, double* acc) {
+  const int bx = db.x - da.x, by = db.y - da.y;
+  const int64_t total = (int64_t)bx * by * (db.z - da.z);
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
-    const int x = (int)(i % wd.x), y = (int)((i / wd.x) % wd.y), z = (int)(i / ((int64_t)wd.x * wd.y));
+    const int x = da.x + (int)(i % bx), y = da.y + (int)((i / bx) % by), z = da.z + (int)(i / ((int64_t)bx * by));
+    const int64_t p = (lo.x + x) + (int64_t)n.x * ((lo.y + y) + (int64_t)n.y * (lo.z + z));
+    acc[p] += (double)win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)];
+  }
+}
+
+// The shell as up to six boxes of window coordinates (below and above the
+// direct box along z, then along y inside its z range, then along x inside its
+// y and z ranges); prefix[k] counts the positions of the boxes before k.
+struct ShellBoxes {
+  Int3 a[6], b[6];
+  int64_t prefix[7];
+  int count;
+};
+
+__global__ __launch_bounds__(kGThreads) void k_fold_mirrored(const float* win, Int3 wd, Int3 lo, Int3 n,
+                                                            ShellBoxes sb, double* acc) {
+  const int64_t total = sb.prefix[sb.count];
+  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
+    int k = 0;
+    while (k + 1 < sb.count && sb.prefix[k + 1] <= i) ++k;
+    const int64_t j = i - sb.prefix[k];
+    const Int3 a = sb.a[k], b = sb.b[k];
+    const int bx = b.x - a.x, by = b.y - a.y;
+    const int x = a.x + (int)(j % bx), y = a.y + (int)((j / bx) % by), z = a.z + (int)(j / ((int64_t)bx * by));
     const int px = mirror_single(lo.x + x, n.x), py = mirror_single(lo.y + y, n.y), pz = mirror_single(lo.z + z, n.z);
-    unsafeAtomicAdd(acc + (px + (int64_t)n.x * (py + (int64_t)n.y * pz)), (double)win[i]);
+    unsafeAtomicAdd(acc + (px + (int64_t)n.x * (py + (int64_t)n.y * pz)), (double)win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)]);
   }
 }
 
 hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream) {
   const int64_t total = (int64_t)wd.x * wd.y * wd.z;
-  const int64_t blocks = std::min<int64_t>((total + kGThreads - 1) / kGThreads, 16384);
-  if (blocks == 0) return hipSuccess;
+  if (total <= 0) return hipSuccess;
+  const Int3 da = {std::max(0, -lo.x), std::max(0, -lo.y), std::max(0, -lo.z)};
+  const Int3 db = {std::min(wd.x, n.x - lo.x), std::min(wd.y, n.y - lo.y), std::min(wd.z, n.z - lo.z)};
+  const bool direct = db.x > da.x && db.y > da.y && db.z > da.z;
+  ShellBoxes sb{};
+  auto add = [&sb](Int3 a, Int3 b) {
+    if (b.x > a.x && b.y > a.y && b.z > a.z) {
+      sb.a[sb.count] = a;
+      sb.b[sb.count] = b;
+      sb.prefix[sb.count + 1] = sb.prefix[sb.count] + (int64_t)(b.x - a.x) * (b.y - a.y) * (b.z - a.z);
+      ++sb.count;
+    }
+  };
+  if (!direct) {
+    add({0, 0, 0}, wd);  // no window position inside the image
+  } else {
+    add({0, 0, 0}, {wd.x, wd.y, da.z});
+    add({0, 0, db.z}, wd);
+    add({0, 0, da.z}, {wd.x, da.y, db.z});
+    add({0, db.y, da.z}, {wd.x, wd.y, db.z});
+    add({0, da.y, da.z}, {da.x, db.y, db.z});
+    add({db.x, da.y, da.z}, {wd.x, db.y, db.z});
+  }
   (void)hipGetLastError();
-  hipLaunchKernelGGL(k_fold_window, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, acc);
+  if (direct) {
+    const int64_t nd = (int64_t)(db.x - da.x) * (db.y - da.y) * (db.z - da.z);
+    const int64_t blocks = std::min<int64_t>((nd + kGThreads - 1) / kGThreads, 16384);
+    hipLaunchKernelGGL(k_fold_direct, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, da, db,
+                       acc);
+  }
+  if (sb.count > 0) {
+    const int64_t ns = sb.prefix[sb.count];
+    const int64_t blocks = std::min<int64_t>((ns + kGThreads - 1) / kGThreads, 16384);
+    hipLaunchKernelGGL(k_fold_mirrored, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, sb, acc);
+  }
   return hipGetLastError();
 }
 
