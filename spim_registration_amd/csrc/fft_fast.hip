@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
-        v[b * S::R0 + t] = buf_load2(r, z * plane + off, row);
+        v[b * S::R0 + t] = buf_load2_nt(r, z * plane + off, row);
       }
   };
   float2 nxt[E];
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
       for (int t = 0; t < S::R0; ++t) {
         const int z = elem_index<N, E, S::R0>(jt, b, t);
         const bool keep = store_col && z >= zlo && z < zhi;
-        buf_store2(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t], row);
+        buf_store2_nt(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t], row);
       }
   }
 }

@@ -28,6 +28,18 @@ namespace mvd {
 
 namespace {
 
+// Spectrum stores that bypass the caches' retention (the block spectra of a
+// view are far larger than the last-level cache and are read back by the next
+// pass only after the whole launch).
+typedef float StreamF2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+  __builtin_nontemporal_store(StreamF2{v.x, v.y}, reinterpret_cast<StreamF2*>(p));
+}
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+  const StreamF2 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF2*>(p));
+  return make_float2(v.x, v.y);
+}
+
 constexpr int kXYThreads = 1024;
 constexpr int kXYM = 128;        // complex samples of a packed row (256 reals)
 constexpr int kXYN = 256;        // rows of a plane, length of the y transforms
@@ -248,13 +260,13 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
   for (int t = 0; t < 16; ++t) {
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
     const float2 d0 = make_float2(__shfl(sdc.x, t), __shfl(sdc.y, t));
-    out[ky * P.pitch + cl] = cl == 0 ? d0 : c0[t];
+    stream_store(&out[ky * P.pitch + cl], cl == 0 ? d0 : c0[t]);
   }
   xy_fft<kXYN>(c1, lds, jy, tw, 1, -1.f, yaddr);
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    out[ky * P.pitch + 64 + cl] = c1[t];
+    stream_store(&out[ky * P.pitch + 64 + cl], c1[t]);
   }
   // the Nyquist column and the padding of this wave's 16 rows: 16 lanes per
   // row, four rows per store instruction (pitch - 128 = 16 columns)
@@ -267,7 +279,7 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
       const float2 d = cscale(csub(dcn[ky], cconj(dcn[(kXYN - ky) & (kXYN - 1)])), 0.5f);
       v = make_float2(d.y, -d.x);
     }
-    out[ky * P.pitch + kXYM + c] = v;
+    stream_store(&out[ky * P.pitch + kXYM + c], v);
   }
 }
 
@@ -300,10 +312,10 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const auto row = in + elem_index<kXYN, 16, SY::R0>(jy, 0, t) * P.pitch;
-    c0[t] = row[cl];
-    c1[t] = row[64 + cl];
+    c0[t] = stream_load(&row[cl]);
+    c1[t] = stream_load(&row[64 + cl]);
   }
-  if (tid < kXYN) nyq[tid] = in[tid * P.pitch + kXYM];
+  if (tid < kXYN) nyq[tid] = stream_load(&in[tid * P.pitch + kXYM]);
   __syncthreads();
   if (cl == 0) {
 #pragma unroll

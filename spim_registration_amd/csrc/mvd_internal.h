@@ -135,6 +135,16 @@ __device__ __forceinline__ void buf_store2(BufRes r, uint32_t byte_off, float2 v
   typedef unsigned int U2 __attribute__((vector_size(8)));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, byte_off, soff, 0);
 }
+// Streaming forms (cache policy nt): spectrum data that is read or written
+// once per pass and is far larger than the caches.
+constexpr int kCachePolicyNT = 2;
+__device__ __forceinline__ float2 buf_load2_nt(BufRes r, uint32_t byte_off, uint32_t soff = 0) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, soff, kCachePolicyNT));
+}
+__device__ __forceinline__ void buf_store2_nt(BufRes r, uint32_t byte_off, float2 v, uint32_t soff = 0) {
+  typedef unsigned int U2 __attribute__((vector_size(8)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, byte_off, soff, kCachePolicyNT);
+}
 #else
 struct BufRes {
   char* p;
@@ -145,6 +155,10 @@ __device__ __forceinline__ float2 buf_load2(BufRes r, uint32_t o, uint32_t so = 
 }
 __device__ __forceinline__ void buf_store2(BufRes r, uint32_t o, float2 v, uint32_t so = 0) {
   *(float2*)(r.p + o + so) = v;
+}
+__device__ __forceinline__ float2 buf_load2_nt(BufRes r, uint32_t o, uint32_t so = 0) { return buf_load2(r, o, so); }
+__device__ __forceinline__ void buf_store2_nt(BufRes r, uint32_t o, float2 v, uint32_t so = 0) {
+  buf_store2(r, o, v, so);
 }
 #endif
 
