@@ -24,6 +24,7 @@ template <int N>
 __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
@@ -40,7 +41,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 #pragma unroll
     for (int t = 0; t < S::R0; ++t) {
       const int y = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = valid ? base[y * P.pitch] : make_float2(0.f, 0.f);
+      v[b * S::R0 + t] = valid ? maybe_stream_load<kStream>(&base[y * P.pitch]) : make_float2(0.f, 0.f);
     }
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
@@ -51,7 +52,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int y = elem_index<N, E, RL>(jt, b, t);
-        if (y >= P.ylo && y < P.yhi) base[y * P.pitch] = v[b * RL + t];
+        if (y >= P.ylo && y < P.yhi) maybe_stream_store<kStream>(&base[y * P.pitch], v[b * RL + t]);
       }
   }
 }
@@ -86,6 +87,7 @@ template <int N>
 __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
@@ -102,7 +104,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
     for (int t = 0; t < S::R0; ++t) {
       const int z = elem_index<N, E, S::R0>(jt, b, t);
       v[b * S::R0 + t] = make_float2(0.f, 0.f);
-      if (valid) v[b * S::R0 + t] = gmem(P.spec)[z * plane + off];
+      if (valid) v[b * S::R0 + t] = maybe_stream_load<kStream>(&gmem(P.spec)[z * plane + off]);
     }
   auto addr = [line](int i) { return i * LP + line; };
   constexpr int RL = last_radix<N>();
@@ -130,7 +132,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
         for (int t = 0; t < S::R0; ++t) {
           const int z = elem_index<N, E, S::R0>(jt, b, t);
-          if (z >= P.zlo && z < P.zhi) gmem(P.spec)[z * plane + off] = v[b * S::R0 + t];
+          if (z >= P.zlo && z < P.zhi) maybe_stream_store<kStream>(&gmem(P.spec)[z * plane + off], v[b * S::R0 + t]);
         }
     }
   } else if (valid) {
@@ -139,7 +141,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int z = elem_index<N, E, RL>(jt, b, t);
-        gmem(P.spec)[z * plane + off] = cscale(v[b * RL + t], P.scale);
+        maybe_stream_store<kStream>(&gmem(P.spec)[z * plane + off], cscale(v[b * RL + t], P.scale));
       }
   }
 }
@@ -158,6 +160,7 @@ template <int N>
 __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
-        v[b * S::R0 + t] = buf_load2_nt(r, z * plane + off, row);
+        v[b * S::R0 + t] = kStream ? buf_load2_nt(r, z * plane + off, row) : buf_load2(r, z * plane + off, row);
       }
   };
   float2 nxt[E];
@@ -223,7 +226,9 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
       for (int t = 0; t < S::R0; ++t) {
         const int z = elem_index<N, E, S::R0>(jt, b, t);
         const bool keep = store_col && z >= zlo && z < zhi;
-        buf_store2_nt(out, keep ? (uint32_t)z * plane + off : kNoLane, v[b * S::R0 + t], row);
+        const uint32_t o = keep ? (uint32_t)z * plane + off : kNoLane;
+        if constexpr (kStream) buf_store2_nt(out, o, v[b * S::R0 + t], row);
+        else buf_store2(out, o, v[b * S::R0 + t], row);
       }
   }
 }
@@ -376,12 +381,12 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
     {
       const float2 bc = cconj(b);
       const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
-      out[k] = cadd(E2, cmul(tw[k], make_float2(D.y, -D.x)));
+      stream_store(&out[k], cadd(E2, cmul(tw[k], make_float2(D.y, -D.x))));
     }
     if (M - k != k && k != 0) {
       const float2 ac = cconj(a);
       const float2 E2 = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
-      out[M - k] = cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x)));
+      stream_store(&out[M - k], cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x))));
     }
   }
   for (int idx = threadIdx.x; idx < L * 16; idx += kThreads) {
@@ -395,7 +400,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
       const float2 E2 = cscale(cadd(a, ac), 0.5f), D = cscale(csub(a, ac), 0.5f);
       v = cadd(E2, cmul(tw[M], make_float2(D.y, -D.x)));
     }
-    gmem(P.spec)[(int64_t)row * P.pitch + M + c] = v;
+    stream_store(&gmem(P.spec)[(int64_t)row * P.pitch + M + c], v);
   }
 }
 
@@ -455,7 +460,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
 #pragma unroll
       for (int e = 0; e < EPT_C; ++e) {
         vals[r][e] = make_float2(0.f, 0.f);
-        if (srow) vals[r][e] = srow[k0 + e * kThreads];
+        if (srow) vals[r][e] = stream_load(&srow[k0 + e * kThreads]);
       }
     }
 #pragma unroll
@@ -468,7 +473,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       float2 val = make_float2(0.f, 0.f);
       if (row < nrows) {
         const int y = g.eff_local.y + row % ny, z = g.eff_local.z + row / ny;
-        val = gmem(P.spec)[((int64_t)z * g.q.y + y) * P.pitch + M];
+        val = stream_load(&gmem(P.spec)[((int64_t)z * g.q.y + y) * P.pitch + M]);
       }
       buf[l * MP + M] = val;
     }

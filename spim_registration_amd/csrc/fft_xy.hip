@@ -28,18 +28,6 @@ namespace mvd {
 
 namespace {
 
-// Spectrum stores that bypass the caches' retention (the block spectra of a
-// view are far larger than the last-level cache and are read back by the next
-// pass only after the whole launch).
-typedef float StreamF2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void stream_store(float2* p, float2 v) {
-  __builtin_nontemporal_store(StreamF2{v.x, v.y}, reinterpret_cast<StreamF2*>(p));
-}
-__device__ __forceinline__ float2 stream_load(const float2* p) {
-  const StreamF2 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF2*>(p));
-  return make_float2(v.x, v.y);
-}
-
 constexpr int kXYThreads = 1024;
 constexpr int kXYM = 128;        // complex samples of a packed row (256 reals)
 constexpr int kXYN = 256;        // rows of a plane, length of the y transforms

@@ -162,6 +162,32 @@ __device__ __forceinline__ void buf_store2_nt(BufRes r, uint32_t o, float2 v, ui
 }
 #endif
 
+// Streaming (nontemporal) access to spectrum data: every pass reads and writes
+// each spectrum value once and a view's block spectra are far larger than the
+// caches, so they are kept from displacing data that is reused.
+typedef float StreamF2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+  const StreamF2 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF2*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+  __builtin_nontemporal_store(StreamF2{v.x, v.y}, reinterpret_cast<StreamF2*>(p));
+}
+// Column passes stream only when a workgroup's segment of a row covers whole
+// 128-byte lines: nontemporal stores of half lines from two workgroups are
+// not merged in L2 and reach HBM as read-modify-writes (measured: the
+// 512-point z pass, 8 columns per workgroup, 4.2 -> 6.4 ms per launch).
+template <bool ON>
+__device__ __forceinline__ float2 maybe_stream_load(const float2* p) {
+  if constexpr (ON) return stream_load(p);
+  else return *p;
+}
+template <bool ON>
+__device__ __forceinline__ void maybe_stream_store(float2* p, float2 v) {
+  if constexpr (ON) stream_store(p, v);
+  else *p = v;
+}
+
 FftPlan make_plan(int n);
 inline int spectrum_pitch(int hx) { return (hx + 15) / 16 * 16; }
 bool fft_size_supported(int n);
