@@ -367,6 +367,9 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     return (g.eff_offset.x + x) + g.dst.x * ((g.eff_offset.y + y - ylo) + g.dst.y * (g.eff_offset.z + zo_));
   };
   // inputs of the epilogue for rows rsub + 4 (c R + i) of half h; lanes run along x
+  // The views and weights are read once per iteration: streaming loads keep
+  // them from displacing the estimate and ratio lines that the forward gathers
+  // of neighbouring blocks (overlapping windows) read again.
   auto load_batch = [&](int h, int c, float (&in0)[R], float (&in1)[R]) {
     const int lane = opaque(threadIdx.x);
     const int x = lane & 255, rsub = lane >> 8;
@@ -379,11 +382,11 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       if (x_live && y >= ylo && y < yhi) {
         const int gi = out_index(x, y, zo);
         if (EPI == kEpiQuotient || EPI == kEpiResidual) {
-          in0[i] = gmem(e.img)[gi];
-          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = gmem(e.weight)[gi];
+          in0[i] = __builtin_nontemporal_load(&gmem(e.img)[gi]);
+          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = __builtin_nontemporal_load(&gmem(e.weight)[gi]);
         } else if (EPI == kEpiUpdate) {
           in0[i] = gmem(e.dst)[gi];
-          if (e.weight_mode != kWeightNone) in1[i] = gmem(e.weight)[gi];
+          if (e.weight_mode != kWeightNone) in1[i] = __builtin_nontemporal_load(&gmem(e.weight)[gi]);
         }
       }
     }
