@@ -502,6 +502,7 @@ int write_tiff_stack(const float* img, bool on_device, int nx, int ny, int nz, i
     // little-endian float32 from host memory: the bytes as they are
     if (!write_all(fd, img, (size_t)data_bytes)) rc = fail("write failed", -3);
     if (::close(fd) != 0 && rc == 0) rc = fail("close failed", -3);
+    if (rc != 0) (void)::unlink(path);
     return rc;
   }
   void* d_in = nullptr;
@@ -564,6 +565,7 @@ int write_tiff_stack(const float* img, bool on_device, int nx, int ny, int nz, i
   }
   if (d_in) (void)hipFree(d_in);
   if (::close(fd) != 0 && rc == 0) rc = fail("close failed", -3);
+  if (rc != 0) (void)::unlink(path);  // no partial file is left behind
   return rc;
 }
 
