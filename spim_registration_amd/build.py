@@ -20,7 +20,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["conv_kernels.hip", "fft_fast.hip", "fft_xy.hip", "pointwise_kernels.hip", "gradient.hip", "session.hip", "jna_abi.hip", "transform.hip", "psf.hip", "sepconv.hip", "export.hip"]
+HIP_SOURCES = ["conv_kernels.hip", "fft_fast.hip", "fft_xy.hip", "pointwise_kernels.hip", "gradient.hip", "session.hip", "jna_abi.hip", "transform.hip", "psf.hip", "sepconv.hip", "export.hip", "staging.hip"]
 HIP_LIB = os.path.join(PKG, "libmvdecon_hip.so")
 SYNTH_LIB = os.path.join(PKG, "libmvd_synth.so")
 

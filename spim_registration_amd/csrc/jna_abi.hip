@@ -208,7 +208,7 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
     k.spec = nullptr;
     ks = &c->kcache.front();
   }
-  JNA_HIP(hipMemcpyAsync(c->block, im, sizeof(float) * nb, hipMemcpyHostToDevice, c->stream));
+  JNA_HIP(staged_h2d(c->block, im, sizeof(float) * nb, c->stream));  // Java array: pageable
   ConvGeom g{};
   g.src = bd;
   g.off = {0, 0, 0};
@@ -224,7 +224,7 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
   JNA_HIP(launch_xy_forward_passes(*sp, g, c->block, kGatherZero, c->spec, c->stream));
   JNA_HIP(launch_z(*sp, c->spec, ks->spec, 1.f, 0, bd.z, c->stream));
   JNA_HIP(launch_yx_inverse_passes(*sp, g, c->spec, e, c->stream, nullptr));
-  JNA_HIP(hipMemcpyAsync(out, c->block, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream));
+  JNA_HIP(staged_d2h(out, c->block, sizeof(float) * nb, c->stream));
   JNA_HIP(hipStreamSynchronize(c->stream));
   return true;
 }

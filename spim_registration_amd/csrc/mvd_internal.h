@@ -236,6 +236,11 @@ hipError_t launch_reduce_stats(const double* part_sum, const float* part_max, in
 hipError_t launch_first_iteration(const float* views, int nviews, int64_t n, float* counts, double* part_sum,
                                   unsigned long long* part_cnt, int nparts, hipStream_t stream);
 hipError_t launch_sum(const float* a, int64_t n, double* part_sum, int nparts, hipStream_t stream);
+// Copies between pageable host arrays and device memory through pinned
+// staging buffers, several host threads and overlapped DMA (staging.hip);
+// they return when the copy is complete.
+hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t stream);
+hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t stream);
 // Output path (export.hip): min/max with Math.min/max semantics, the 16-bit
 // range widening and the TIFF stack writer (img in device or host memory).
 hipError_t launch_min_max(const float* img, int64_t n, float* minmax, hipStream_t stream);

@@ -146,7 +146,7 @@ int sep_convolve(int size, float* image, const float* kx, const float* ky, const
   hipError_t e = hipMalloc(&a, sizeof(float) * n);
   if (e == hipSuccess) e = hipMalloc(&b, sizeof(float) * n);
   if (e == hipSuccess) e = hipMalloc(&k, sizeof(float) * 3 * size);
-  if (e == hipSuccess) e = hipMemcpy(a, image, sizeof(float) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = staged_h2d(a, image, sizeof(float) * n, nullptr);  // Java array: pageable
   if (e == hipSuccess && cx) e = hipMemcpy(k, kx, sizeof(float) * size, hipMemcpyHostToDevice);
   if (e == hipSuccess && cy) e = hipMemcpy(k + size, ky, sizeof(float) * size, hipMemcpyHostToDevice);
   if (e == hipSuccess && cz) e = hipMemcpy(k + 2 * size, kz, sizeof(float) * size, hipMemcpyHostToDevice);
@@ -165,7 +165,7 @@ int sep_convolve(int size, float* image, const float* kx, const float* ky, const
     }
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess) e = hipMemcpy(image, res, sizeof(float) * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = staged_d2h(image, res, sizeof(float) * n, nullptr);
   for (float* p : {a, b, k})
     if (p) (void)hipFree(p);
   (void)hipSetDevice(prev);
