@@ -267,7 +267,7 @@ extern "C" int mvd_extract_psf(int device, const float* img, int nx, int ny, int
   if (e == hipSuccess) e = hipMalloc(&d_psf, sizeof(float) * m);
   if (e == hipSuccess && nloc > 0) e = hipMalloc(&d_loc, sizeof(double) * 3 * nloc);
   if (e == hipSuccess && transformed_out) e = hipMalloc(&d_out, sizeof(float) * mt);
-  if (e == hipSuccess) e = hipMemcpy(d_img, img, sizeof(float) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = staged_h2d(d_img, img, sizeof(float) * n, nullptr);
   if (e == hipSuccess && nloc > 0) e = hipMemcpy(d_loc, locations, sizeof(double) * 3 * nloc, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     ExtractArgs a{};

@@ -1113,12 +1113,9 @@ int mvd_add_view(mvd_session* s, const float* img, const float* weight, const fl
   const int v = (int)s->views.size();
   if (weight) {
     MVD_CHECK(s->d_weights.ensure(sizeof(float) * s->n * s->nviews));
-    MVD_CHECK(hipMemcpyAsync(s->d_weights.as<float>() + v * s->n, weight, sizeof(float) * s->n,
-                             hipMemcpyHostToDevice, s->stream));
+    MVD_CHECK(staged_h2d(s->d_weights.as<float>() + v * s->n, weight, sizeof(float) * s->n, s->stream));
   }
-  MVD_CHECK(hipMemcpyAsync(s->d_views.as<float>() + v * s->n, img, sizeof(float) * s->n, hipMemcpyHostToDevice,
-                           s->stream));
-  MVD_CHECK(hipStreamSynchronize(s->stream));
+  MVD_CHECK(staged_h2d(s->d_views.as<float>() + v * s->n, img, sizeof(float) * s->n, s->stream));
   register_view(s, psf, kx, ky, kz, weight != nullptr);
   return 0;
 }
@@ -1155,7 +1152,7 @@ int mvd_add_view_transformed(mvd_session* s, const float* raw, int rx, int ry, i
   if (const char* msg = check_transform_args(a)) MVD_REQUIRE(false, msg);
   DeviceBuf d_raw;
   MVD_CHECK(d_raw.ensure(sizeof(float) * (size_t)rx * ry * rz));
-  MVD_CHECK(hipMemcpyAsync(d_raw.p, raw, sizeof(float) * (size_t)rx * ry * rz, hipMemcpyHostToDevice, s->stream));
+  MVD_CHECK(staged_h2d(d_raw.p, raw, sizeof(float) * (size_t)rx * ry * rz, s->stream));
   a.raw = d_raw.as<float>();
   s->timer.begin(s->stream);
   MVD_CHECK(launch_transform_view(a, s->stream));
@@ -1338,7 +1335,7 @@ int mvd_init_psi(mvd_session* s, const float* psi0, double* avg_out) {
   MVD_REQUIRE((int)s->views.size() == s->nviews, "add all views first");
   DeviceGuard guard(s->device);
   if (psi0) {
-    MVD_CHECK(hipMemcpyAsync(s->d_psi.p, psi0, sizeof(float) * s->n, hipMemcpyHostToDevice, s->stream));
+    MVD_CHECK(staged_h2d(s->d_psi.p, psi0, sizeof(float) * s->n, s->stream));
     MVD_CHECK(hipStreamSynchronize(s->stream));
     if (avg_out) *avg_out = std::nan("");
   } else {
@@ -1437,19 +1434,18 @@ int mvd_get_psi(mvd_session* s, float* out, int masked) {
   MVD_REQUIRE(s && out, "NULL argument");
   MVD_REQUIRE(s->psi_ready, "mvd_init_psi first");
   DeviceGuard guard(s->device);
-  MVD_CHECK(hipMemcpyAsync(out, s->d_psi.p, sizeof(float) * s->n, hipMemcpyDeviceToHost, s->stream));
-  MVD_CHECK(hipStreamSynchronize(s->stream));
-  if (masked) {
-    // counts of views with data into the ratio buffer (scratch between iterations)
-    double sum;
-    unsigned long long cnt;
-    int rc = first_iteration_sums(s, s->d_ratio.as<float>(), &sum, &cnt);
-    if (rc) return rc;
-    std::vector<float> counts(s->n);
-    MVD_CHECK(hipMemcpy(counts.data(), s->d_ratio.p, sizeof(float) * s->n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < s->n; ++i)
-      if (counts[i] == 0.f) out[i] = 0.f;
+  if (!masked) {
+    MVD_CHECK(staged_d2h(out, s->d_psi.p, sizeof(float) * s->n, s->stream));
+    return 0;
   }
+  // the counts of views with data into the ratio buffer (scratch between
+  // iterations), then psi where a view has data, in place, and one copy
+  double sum;
+  unsigned long long cnt;
+  int rc = first_iteration_sums(s, s->d_ratio.as<float>(), &sum, &cnt);
+  if (rc) return rc;
+  MVD_CHECK(launch_mask(s->d_psi.as<float>(), s->d_ratio.as<float>(), s->d_ratio.as<float>(), s->n, s->stream));
+  MVD_CHECK(staged_d2h(out, s->d_ratio.p, sizeof(float) * s->n, s->stream));
   return 0;
 }
 
@@ -1488,7 +1484,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
   const float* psrc = s->d_psi.as<float>();
   if (psi) {
     MVD_CHECK(s->g_psi.ensure(sizeof(float) * s->n));
-    MVD_CHECK(hipMemcpyAsync(s->g_psi.p, psi, sizeof(float) * s->n, hipMemcpyHostToDevice, st));
+    MVD_CHECK(staged_h2d(s->g_psi.p, psi, sizeof(float) * s->n, st));
     psrc = s->g_psi.as<float>();
   }
   if (want_grad_psi) {
@@ -1633,7 +1629,7 @@ int mvd_get_grad_psi(mvd_session* s, float* out) {
   MVD_REQUIRE(s && out, "NULL argument");
   MVD_REQUIRE(s->g_gpsi_ready, "no estimate gradient: call mvd_nll_grad_device with want_grad_psi first");
   DeviceGuard guard(s->device);
-  MVD_CHECK(hipMemcpyAsync(out, s->g_gpsi.p, sizeof(float) * s->n, hipMemcpyDeviceToHost, s->stream));
+  MVD_CHECK(staged_d2h(out, s->g_gpsi.p, sizeof(float) * s->n, s->stream));
   MVD_CHECK(hipStreamSynchronize(s->stream));
   return 0;
 }

@@ -281,7 +281,7 @@ extern "C" int mvd_transform_view(int device, const float* raw, int rx, int ry, 
     fail("out of device memory (image)", -2);
   } else if (out_weight && hipMalloc(&d_w, sizeof(float) * n) != hipSuccess) {
     fail("out of device memory (weight)", -2);
-  } else if (out_img && hipMemcpy(d_raw, raw, sizeof(float) * nr, hipMemcpyHostToDevice) != hipSuccess) {
+  } else if (out_img && staged_h2d(d_raw, raw, sizeof(float) * nr, nullptr) != hipSuccess) {
     fail("copy of the raw view failed", -2);
   } else {
     a.raw = d_raw;
@@ -289,8 +289,8 @@ extern "C" int mvd_transform_view(int device, const float* raw, int rx, int ry, 
     a.out_w = d_w;
     hipError_t e = launch_transform_view(a, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess && out_img) e = hipMemcpy(out_img, d_img, sizeof(float) * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && out_weight) e = hipMemcpy(out_weight, d_w, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_img) e = staged_d2h(out_img, d_img, sizeof(float) * n, nullptr);
+    if (e == hipSuccess && out_weight) e = staged_d2h(out_weight, d_w, sizeof(float) * n, nullptr);
     if (e != hipSuccess) fail(std::string("transform failed: ") + hipGetErrorString(e), -2);
   }
   if (d_raw) (void)hipFree(d_raw);
