@@ -260,6 +260,16 @@ def run_c4(args):
     e1 = psf_err()
     log(f"{args.refine_steps} refinement steps in {rsec:.2f} s; PSF rel. error {e0:.4f} -> {e1:.4f}; "
         f"NLL {hist[0].nll:.6e} -> {hist[-1].nll:.6e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's float32 gradient (NLL, d/dpsi, d/dP) of all views at the full size
+        t0 = time.perf_counter()
+        O.nll_grad(list(ds.views), kernels, psi)
+        csec = time.perf_counter() - t0
+        cpu = dict(value=round(n * nv / csec / 1e9, 5), unit="Gvoxel·view·grad-eval/s", cores=O.default_threads(),
+                   kind="port", sample=f"oracle float32 gradient of the 4 views (256^3, PSF 31^3), one evaluation, "
+                                       f"{csec:.2f} s")
+        log(f"cpu baseline {cpu['value']} ({csec:.1f} s)")
     d.close()
     # path model per view and gradient evaluation: forward convolution, PSF
     # correlation and adjoint convolution over the window img + K - 1, plus the
@@ -297,6 +307,7 @@ def run_c4(args):
                        "ms_per_step": round(1e3 * rsec / max(1, args.refine_steps), 2),
                        "psf_rel_err_start": round(e0, 5), "psf_rel_err_end": round(e1, 5),
                        "nll_first": hist[0].nll if hist else None, "nll_last": hist[-1].nll if hist else None},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -489,6 +500,15 @@ def run_dog(args):
         res[n] = dict(ms=round(1e3 * sec, 3), gvox_s=round(w * h * d / sec / 1e9, 3), max_rel_err=err)
         log(f"convolve_{n} (sigma {sigma}): {1e3 * sec:.2f} ms per call -> {w * h * d / sec / 1e9:.2f} Gvoxel/s "
             f"(PCIe included), max rel err {err:.2e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        slab = img[:128]
+        k15 = SC.padded_kernel(SC.gaussian_kernel(2.0), 15)
+        t0 = time.perf_counter()
+        SC.separable_convolve(slab, k15, k15, k15, SC.EXTEND_BORDER_PIXELS)
+        csec = time.perf_counter() - t0
+        cpu = dict(value=round(slab.size / csec / 1e9, 4), unit="Gvoxel/s", cores=1, kind="port",
+                   sample=f"numpy float64 oracle, 15 taps on x, y, z, 128 of the 256 planes, {csec:.2f} s")
     out = {
         "metric": "separable Gaussian convolution (convolve_N hook), voxels per second, host arrays (PCIe included)",
         "value": res[15]["gvox_s"], "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -496,7 +516,7 @@ def run_dog(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "dog: 512x512x256 f32, Gaussian on x, y, z, border extension; value at 15 taps"},
         "sizes": {str(k): v for k, v in res.items()},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
