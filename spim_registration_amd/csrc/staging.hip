@@ -5,6 +5,9 @@
 // several host threads while the DMA engine moves the previous chunk, so the
 // single-threaded pageable copy of the runtime is not the bound.  Small
 // copies use the runtime directly.  Each call returns when its copy is done.
+// The two 16 MiB pinned buffers of a device are allocated at its first large
+// copy and held until the process ends; large copies on one device are
+// serialised by a mutex (the JNA hooks are called from one thread per device).
 #include <algorithm>
 #include <cstring>
 #include <mutex>
