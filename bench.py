@@ -763,6 +763,22 @@ def main():
         log(f"cpu baseline {cpu['value']:.4f} {UNIT} ({time.perf_counter() - t:.1f} s)")
     d.close()
 
+    # a plain device-to-device copy of 1 GiB beside the roofline: the practical
+    # HBM rate of a streaming read + write on this device
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    dst.copy_(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
+    log(f"copy probe: {copy_gbs:.0f} GB/s (read + write of 1 GiB)")
+
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -787,6 +803,7 @@ def main():
         "path_roofline": {"achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(path_gbs / HBM_PEAK_GBS, 4),
                           "model": "8P + 48H + 20E bytes per block, view and iteration"},
+        "copy_probe": {"GB/s": round(copy_gbs, 1), "what": "torch device-to-device copy of 1 GiB (read + write)"},
         "kernels": {k: {"avg_us": round(v["avg_us"], 2), "GB/s": round(v["gbs"], 1),
                         "share": round(v["ms_total"] / sum(x["ms_total"] for x in per_class.values()), 3)}
                     for k, v in per_class.items()},
