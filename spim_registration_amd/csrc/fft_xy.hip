@@ -430,6 +430,9 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     }
   };
   float p0[R], p1[R];  // first batch of a half, loaded ahead of the transform where registers allow
+  // the first half's batch can be in flight during its transform only for the
+  // one-input epilogues: with two inputs the held registers would spill
+  constexpr bool kEarlyLoad = EPI == kEpiQuotient || EPI == kEpiStore;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     if (h == 1) {
@@ -437,6 +440,8 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       put_rows(c0, 0, 1);
       put_rows(c1, 64, 1);
       load_batch(1, 0, p0, p1);  // no plane values are held in registers any more
+    } else if constexpr (kEarlyLoad) {
+      load_batch(0, 0, p0, p1);  // in flight during the first half's transform
     }
     __syncthreads();
     // pack X[k], X[M - k] into Z[k] = E + i O
@@ -454,7 +459,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     xy_rows_fft(lds, tw, 1.f);
     __syncthreads();
     // epilogue over this half's output rows; lanes run along x
-    if (h == 0) load_batch(0, 0, p0, p1);
+    if (!kEarlyLoad && h == 0) load_batch(0, 0, p0, p1);
     run_batch(h, 0, p0, p1);
 #pragma unroll
     for (int c = 1; c < 32 / R; ++c) {
