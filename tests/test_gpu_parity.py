@@ -43,7 +43,9 @@ def test_device_query(jna):
     # power-of-two FFT lengths take the register-resident kernels
     ((64, 64, 64), (13, 11, 9)), ((128, 128, 128), (31, 19, 19)), ((64, 256, 256), (51, 31, 31)),
     ((64, 64, 512), (9, 9, 41)), ((32, 64, 1024), (5, 7, 9)), ((512, 64, 64), (81, 9, 9)),
-    ((256, 128, 128), (3, 3, 3))])
+    ((256, 128, 128), (3, 3, 3)),
+    # odd sizes above the staging threshold: two 16 MiB chunks, the second ragged
+    ((150, 170, 190), (7, 5, 9))])
 def test_jna_inplace_interior_matches_linear_convolution(jna, bshape, kshape):
     rng = np.random.default_rng(sum(bshape))
     blk = rng.random(bshape).astype(np.float32)
