@@ -92,7 +92,7 @@ def block_effective_sizes(img, ker, bs):
 
 
 def cpu_baseline(ds, w, threads):
-    """Oracle float32 path: one view-iteration of the workload (all 18 blocks)."""
+    """Oracle float32 path: one view-iteration of the workload (all its blocks)."""
     from oracle import oracle as O
     views, psfs = ds.views[:1], ds.psfs[:1]
     blk = w["block"]
@@ -103,9 +103,15 @@ def cpu_baseline(ds, w, threads):
     t2 = time.perf_counter()
     it = (t2 - t1) - (t1 - t0)
     n = int(np.prod(w["size_xyz"]))
+    sx, sy, sz = w["size_xyz"]
+    if blk:
+        nb = int(np.prod([-(-s_ // (b_ - k_ + 1)) for s_, b_, k_ in zip(w["size_xyz"], blk, w["k_xyz"])]))
+        where = f"{nb} blocks of {blk[0]}x{blk[1]}x{blk[2]}"
+    else:
+        where = "one window of img + K - 1"
     return dict(value=n / it / 1e9, unit=UNIT, cores=threads, kind="port",
-                sample=f"oracle float32 CPU path, 1 view x 1 iteration of the 512x512x256 workload "
-                       f"(18 blocks of 256^3), {it:.2f} s, setup excluded")
+                sample=f"oracle float32 CPU path, 1 view x 1 iteration of the {sx}x{sy}x{sz} workload "
+                       f"({where}), {it:.2f} s, setup excluded")
 
 
 def run_c3(args):
