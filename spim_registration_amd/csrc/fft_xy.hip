@@ -178,7 +178,12 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
         int gy = g.off.y + y;
         const bool y_in = gy >= 0 && gy < g.src.y;
         if (MODE == kGatherMirror) {
-          if (!y_in) gy = mirror_single(gy, g.src.y);
+          // one reflection covers every window whose halo is shorter than the
+          // image; the general (periodic) form handles the rest
+          if (!y_in) {
+            const int r = gy < 0 ? -gy : 2 * g.src.y - 2 - gy;
+            gy = (r >= 0 && r < g.src.y) ? r : mirror_single(gy, g.src.y);
+          }
           val = gmem(P.src)[gx + g.src.x * (gy + g.src.y * gz)];
         } else if (z_in && y_in && x_in) {
           val = gmem(P.src)[gx + g.src.x * (gy + g.src.y * gz)];
