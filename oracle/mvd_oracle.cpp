@@ -426,6 +426,11 @@ double first_iteration(const std::vector<const T*>& imgs, int64_t n, int norm_th
   return s.sum() / (double)count;
 }
 
+// java.lang.Math.max(double, double): NaN when either argument is NaN
+// (std::max returns the first argument instead).  The change statistics use
+// it (deconv/MVDeconvolution.java:647,677,454).
+inline double java_max(double a, double b) { return a != a ? a : b != b ? b : std::max(a, b); }
+
 // MVDeconvolution.computeQuotient, deconv/MVDeconvolution.java:494-546.
 template <class T>
 inline T quotient(T blurred, T img) {
@@ -513,7 +518,7 @@ void deconvolve(const RunArgs& a, const std::vector<const T*>& imgs, const std::
             psi[i] = next;
             const T change = std::abs(next - last);
             sum_change += change;
-            max_change = std::max(max_change, (double)change);
+            max_change = java_max(max_change, (double)change);
           }
           psum[p] = sum_change;
           pmax[p] = max_change;
@@ -522,7 +527,7 @@ void deconvolve(const RunArgs& a, const std::vector<const T*>& imgs, const std::
       double sum_change = 0, max_change = -1;
       for (size_t p = 0; p < portions.size(); ++p) {
         sum_change += psum[p];
-        max_change = std::max(max_change, pmax[p]);
+        max_change = java_max(max_change, pmax[p]);
       }
       if (stats) {
         stats[(it * a.nviews + v) * 2 + 0] = sum_change;
@@ -567,10 +572,13 @@ Box<T> make_box(Dims d, Dims dk, int threads) {
   return b;
 }
 
+// `variant` != 0 plants a known mistake for the tests that show the checks of
+// this function can fail: 1 drops the mirrored shell of the fold (window
+// positions outside the image contribute nothing to dL/dpsi).
 template <class T>
 double nll_grad(const std::vector<const T*>& imgs, const std::vector<const T*>& weights, const T* psi, Dims d,
                 const std::vector<const T*>& psfs, const std::vector<Dims>& kd, T* grad_psi,
-                const std::vector<T*>& grad_psf, int threads) {
+                const std::vector<T*>& grad_psf, int threads, int variant = 0) {
   using C = std::complex<T>;
   const int64_t n = d.n();
   std::vector<double> gpsi(n, 0.0);
@@ -639,8 +647,9 @@ double nll_grad(const std::vector<const T*>& imgs, const std::vector<const T*>& 
     for (int64_t z = 0; z < d.z + k.z - 1; ++z)
       for (int64_t y = 0; y < d.y + k.y - 1; ++y)
         for (int64_t x = 0; x < d.x + k.x - 1; ++x) {
-          const int64_t px = mirror_index(x - cx, d.x), py = mirror_index(y - cy, d.y),
-                        pz = mirror_index(z - cz, d.z);
+          const int64_t ix = x - cx, iy = y - cy, iz = z - cz;
+          if (variant == 1 && (ix < 0 || ix >= d.x || iy < 0 || iy >= d.y || iz < 0 || iz >= d.z)) continue;
+          const int64_t px = mirror_index(ix, d.x), py = mirror_index(iy, d.y), pz = mirror_index(iz, d.z);
           gpsi[px + d.x * (py + d.y * pz)] += (double)(corr[x + qx * (y + qy * z)] * scale);
         }
   }
@@ -650,6 +659,10 @@ double nll_grad(const std::vector<const T*>& imgs, const std::vector<const T*>& 
 
 // L for one view in double with the blur given (b = blur + h * shift_j psi):
 // central differences of single PSF voxels, each an O(n) rank-one update.
+// L(+) - L(-) is summed voxel by voxel as w (2 h s - y (log1p(h s / b) -
+// log1p(-h s / b))), the same difference without subtracting two full sums
+// (which loses about log10(L / (h dL)) digits); a voxel where either blur is
+// <= 0 falls back to its two separate terms, each skipped when <= 0.
 double nll_view_fd(const double* img, const double* weight, const double* blur, const double* psi, Dims d,
                    Dims k, const int64_t* voxels, int nvox, double h, double* out, int threads) {
   const int64_t cx = k.x / 2, cy = k.y / 2, cz = k.z / 2;
@@ -657,29 +670,29 @@ double nll_view_fd(const double* img, const double* weight, const double* blur, 
     const int64_t j = voxels[t];
     const int64_t jx = j % k.x, jy = (j / k.x) % k.y, jz = j / (k.x * k.y);
     // per-z-slice partial sums, each slice written by one thread, summed in order
-    std::vector<double> sp(d.z, 0.0), sm(d.z, 0.0);
+    std::vector<double> sp(d.z, 0.0);
     parallel_for(d.z, threads, [&](int64_t zb, int64_t ze) {
       for (int64_t z = zb; z < ze; ++z) {
-        double ap = 0, am = 0;
+        double diff = 0;
         for (int64_t y = 0; y < d.y; ++y)
           for (int64_t x = 0; x < d.x; ++x) {
             const int64_t i = x + d.x * (y + d.y * z);
             const double s = extended(psi, d, OOB_MIRROR, x + cx - jx, y + cy - jy, z + cz - jz);
             const double w = weight ? weight[i] : 1.0;
-            const double bp = blur[i] + h * s, bm = blur[i] - h * s;
-            if (bp > 0) ap += w * (bp - img[i] * std::log(bp));
-            if (bm > 0) am += w * (bm - img[i] * std::log(bm));
+            const double b = blur[i], hs = h * s, bp = b + hs, bm = b - hs;
+            if (bp > 0 && bm > 0) {
+              diff += w * (2.0 * hs - img[i] * (std::log1p(hs / b) - std::log1p(-hs / b)));
+            } else {
+              if (bp > 0) diff += w * (bp - img[i] * std::log(bp));
+              if (bm > 0) diff -= w * (bm - img[i] * std::log(bm));
+            }
           }
-        sp[z] = ap;
-        sm[z] = am;
+        sp[z] = diff;
       }
     });
-    double lp = 0, lm = 0;
-    for (int64_t z = 0; z < d.z; ++z) {
-      lp += sp[z];
-      lm += sm[z];
-    }
-    out[t] = (lp - lm) / (2 * h);
+    double dl = 0;
+    for (int64_t z = 0; z < d.z; ++z) dl += sp[z];
+    out[t] = dl / (2 * h);
   }
   return 0;
 }
@@ -833,14 +846,14 @@ int orc_nll_view_fd_f64(const double* img, const double* weight, const double* b
 #define ORC_GRAD(T, SUF)                                                                               \
   int orc_nll_grad_##SUF(int nviews, const int64_t* dims, const T* const* imgs, const T* const* weights,  \
                          const T* psi, const T* const* psfs, const int64_t* kdims, T* grad_psi,          \
-                         T* const* grad_psf, double* nll, int threads) {                                 \
+                         T* const* grad_psf, double* nll, int threads, int variant) {                    \
     ORC_TRY({                                                                                          \
       std::vector<const T*> im(imgs, imgs + nviews), w, ps(psfs, psfs + nviews);                       \
       if (weights) w.assign(weights, weights + nviews);                                                \
       std::vector<Dims> kd;                                                                            \
       for (int v = 0; v < nviews; ++v) kd.push_back({kdims[3 * v], kdims[3 * v + 1], kdims[3 * v + 2]}); \
       std::vector<T*> gp(grad_psf, grad_psf + nviews);                                                 \
-      *nll = nll_grad(im, w, psi, {dims[0], dims[1], dims[2]}, ps, kd, grad_psi, gp, threads);          \
+      *nll = nll_grad(im, w, psi, {dims[0], dims[1], dims[2]}, ps, kd, grad_psi, gp, threads, variant); \
     })                                                                                                 \
   }
 ORC_GRAD(float, f32)

@@ -210,6 +210,72 @@ def next_value(last, integral, weight, lam: float):
     return out
 
 
+# Weight types of the session API (the reference's WeightType,
+# deconv/ProcessForDeconvolution.java:81; AS_GIVEN = weights handed to the
+# operator as they are, LOAD_WEIGHTS' normalisation left to the caller).
+W_NONE, W_AS_GIVEN, W_PRECOMPUTED, W_VIRTUAL = 0, 1, 2, 3
+
+
+def overlap_counts(raw, norm_threads: int):
+    """WeightNormalizer.process (deconv/WeightNormalizer.java:73-116): per
+    portion of 2*numThreads the minimum number of views with weight > 0 and
+    the average count; returns (min over portions, mean of portion averages),
+    each then raised to at least 1 (ProcessForDeconvolution.java:349-350)."""
+    cnt = np.zeros(raw[0].size, np.int64)
+    for w in raw:
+        cnt += (np.ravel(w) > 0)
+    mins, avgs = [], []
+    for start, size in divide_portions(cnt.size, 2 * norm_threads):
+        part = cnt[start:start + size]
+        mins.append(int(part.min()) if size else len(raw))
+        avgs.append(part.sum() / size if size else float("nan"))
+    return max(1, min(mins)), max(1.0, sum(avgs) / len(avgs))  # summed in portion order
+
+
+def effective_weights(raw, mode: int, osem_index: int = 0, osem_value: float = 1.0, norm_threads: int = 8,
+                      shape=None):
+    """The weight each view's update reads (computeFinalValues,
+    deconv/MVDeconvolution.java:603-680), float32 per view, and the OSEM factor.
+
+    PRECOMPUTED: WeightNormalizer.ApplyDirectly (WeightNormalizer.java:138-185):
+      w_v = (float)(w_v / sum_u w_u), the double sum in view order, no guard
+      (0/0 gives NaN).
+    VIRTUAL: ComputeSumImage (:187-253) keeps s = sum > 1 ? (float)sum : 1 and
+      NormalizingRandomAccess.get (weights/NormalizingRandomAccess.java:56-66)
+      reads (float)Math.min(1, (double)w / s * osem).
+    OSEM factor (ProcessForDeconvolution.java:343-356): index 1 = minimum
+      overlap, 2 = average overlap, 3 = the given value, 0 = 1; without weights
+      the overlap counts stay 0.
+    adjustForOSEM (:372-405), if the factor is not 1: PRECOMPUTED / AS_GIVEN
+      w = Math.min(1, w * (float)osem) in float (NaN stays NaN); NO_WEIGHTS the
+      constant 1 becomes Math.min(1, 1 * (float)osem).
+    Returns (list of float32 arrays, osem)."""
+    if mode == W_NONE:
+        mn, avg = 0, 0.0
+    elif mode in (W_PRECOMPUTED, W_VIRTUAL):
+        mn, avg = overlap_counts(raw, norm_threads)
+    else:
+        mn, avg = 1, 1.0
+    osem = {0: 1.0, 1: float(mn), 2: float(avg), 3: float(osem_value)}[osem_index]
+    f1 = np.float32(1)
+    if mode == W_NONE:
+        c = f1 if osem == 1.0 else np.minimum(f1, f1 * np.float32(osem))
+        return [np.full(shape, c, np.float32) for _ in range(len(raw) if raw else 1)], osem
+    ws = [np.asarray(w, np.float32) for w in raw]
+    total = np.zeros(ws[0].shape, np.float64)
+    for w in ws:  # view order, as the cursors are read
+        total += w.astype(np.float64)
+    if mode == W_VIRTUAL:
+        ssum = np.where(total > 1, total.astype(np.float32), f1).astype(np.float64)
+        return [np.minimum(1.0, w.astype(np.float64) / ssum * osem).astype(np.float32) for w in ws], osem
+    if mode == W_PRECOMPUTED:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            ws = [(w.astype(np.float64) / total).astype(np.float32) for w in ws]
+    if osem != 1.0:
+        ws = [np.minimum(f1, w * np.float32(osem)) for w in ws]  # np.minimum keeps NaN, as Math.min
+    return ws, osem
+
+
 def deconvolve(views, kernels, psf_type: int, iterations: int, lam: float, weights=None,
                block_xyz=None, norm_threads: int = 8, threads: int | None = None, psi0=None,
                return_kernels: bool = False):
@@ -246,12 +312,14 @@ def _declare_grad(L):
     L.orc_nll_view_fd_f64.argtypes = [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_double, _P, ctypes.c_int]
     for suf in ("f32", "f64"):
         getattr(L, f"orc_nll_grad_{suf}").argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                                      ctypes.c_int]
+                                                      ctypes.c_int, ctypes.c_int]
 
 
-def nll_grad(views, psfs, psi, weights=None, threads=None):
+def nll_grad(views, psfs, psi, weights=None, threads=None, variant: int = 0):
     """Poisson NLL of the forward model b_v = E_mirror(psi) * P_v and its
-    gradient.  Returns (nll, grad_psi, [grad_psf_v])."""
+    gradient.  Returns (nll, grad_psi, [grad_psf_v]).  variant != 0 plants a
+    known mistake (1: the mirrored shell of the psi fold dropped); only the
+    tests that show their checks catch it use it."""
     L = lib()
     _declare_grad(L)
     dtype = np.dtype(psi.dtype)
@@ -268,7 +336,7 @@ def nll_grad(views, psfs, psi, weights=None, threads=None):
     nll = ctypes.c_double()
     _check(getattr(L, f"orc_nll_grad_{_dt(dtype)}")(
         len(views), _ptr(_dims(psi)), _ptr_array(views), w, _ptr(psi), _ptr_array(psfs), _ptr(kd), _ptr(gpsi),
-        _ptr_array(gpsf), ctypes.byref(nll), threads or default_threads()))
+        _ptr_array(gpsf), ctypes.byref(nll), threads or default_threads(), variant))
     return nll.value, gpsi, gpsf
 
 

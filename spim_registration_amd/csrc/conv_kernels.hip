@@ -254,7 +254,6 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
   lds_fft(buf, lp, B, P.plan, tw, 2, 1.f);
   const float* fbuf = reinterpret_cast<const float*>(buf);
   const EpilogueArgs& e = P.e;
-  const double inv_lambda = rl_inv_lambda(e.lambda);
   double my_sum = 0.0;
   float my_max = -1.f;
   const int nx = g.eff_size.x;
@@ -281,25 +280,25 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
     } else if (e.mode == kEpiQuotient) {
       gmem(e.dst)[gi] = rl_quotient(v, gmem(e.img)[gi]);
     } else {
-      float w = 1.f;
+      float w = e.wconst;
       if (e.weight_mode == kWeightArray) {
         w = gmem(e.weight)[gi];
       } else if (e.weight_mode == kWeightVirtual) {
         const double r = (double)gmem(e.weight)[gi] / (double)gmem(e.wsum)[gi];
-        w = (float)fmin(1.0, r * e.osem);
+        w = (float)java_min(1.0, r * e.osem);
       }
       const float last = gmem(e.dst)[gi];
-      const float next = rl_next_value(last, v, w, e.lambda, inv_lambda);
+      const float next = rl_next_value(last, v, w, e.lambda);
       gmem(e.dst)[gi] = next;
       const float change = fabsf(next - last);
       my_sum += (double)change;
-      my_max = fmaxf(my_max, change);
+      my_max = java_maxf(my_max, change);
     }
   }
   if (e.mode == kEpiUpdate || e.mode == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
-      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+      my_max = java_maxf(my_max, __shfl_xor(my_max, o));
     }
     const int wid = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0) {
@@ -312,7 +311,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
       float m = -1.f;
       for (int i = 0; i < kThreads / 64; ++i) {
         s += red_sum[i];
-        m = fmaxf(m, red_max[i]);
+        m = java_maxf(m, red_max[i]);
       }
       gmem(e.part_sum)[bx] = s;
       gmem(e.part_max)[bx] = m;

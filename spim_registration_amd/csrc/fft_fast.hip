@@ -507,7 +507,6 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   __syncthreads();
   const float* fbuf = reinterpret_cast<const float*>(buf);
   const EpilogueArgs& e = P.e;
-  const double inv_lambda = rl_inv_lambda(e.lambda);
   double my_sum = 0.0;
   float my_max = -1.f;
   const int nx = g.eff_size.x;
@@ -537,7 +536,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
           if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[r][ee] = gmem(e.weight)[gi];
         } else if (EPI == kEpiUpdate) {
           in0[r][ee] = gmem(e.dst)[gi];
-          if (e.weight_mode != kWeightNone) in1[r][ee] = gmem(e.weight)[gi];
+          in1[r][ee] = e.weight_mode != kWeightNone ? gmem(e.weight)[gi] : e.wconst;
         }
       }
     }
@@ -568,21 +567,21 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
         float w = in1[r][ee];
         if (e.weight_mode == kWeightVirtual) {
           const double rr = (double)in1[r][ee] / (double)gmem(e.wsum)[gi];
-          w = (float)fmin(1.0, rr * e.osem);
+          w = (float)java_min(1.0, rr * e.osem);
         }
         const float last = in0[r][ee];
-        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
+        const float next = rl_next_value(last, val, w, e.lambda);
         gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
-        my_max = fmaxf(my_max, change);
+        my_max = java_maxf(my_max, change);
       }
     }
   }
   if (EPI == kEpiUpdate || EPI == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
-      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+      my_max = java_maxf(my_max, __shfl_xor(my_max, o));
     }
     const int wid = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0) {
@@ -595,7 +594,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
       float m = -1.f;
       for (int i = 0; i < kThreads / 64; ++i) {
         s += red_sum[i];
-        m = fmaxf(m, red_max[i]);
+        m = java_maxf(m, red_max[i]);
       }
       gmem(e.part_sum)[bx] = s;
       gmem(e.part_max)[bx] = m;

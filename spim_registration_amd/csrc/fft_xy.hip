@@ -294,7 +294,6 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int tid = threadIdx.x;
   const ConvGeom& g = P.g;
   const EpilogueArgs& e = P.e;
-  const double inv_lambda = rl_inv_lambda(e.lambda);
   const int zo = bx;                 // output plane inside the effective box
   const int z = g.eff_local.z + zo;  // plane of the FFT box
   if (tid < kXYN) tw[tid] = gmem(P.tw)[tid];
@@ -391,7 +390,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
           if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[i] = __builtin_nontemporal_load(&gmem(e.weight)[gi]);
         } else if (EPI == kEpiUpdate) {
           in0[i] = gmem(e.dst)[gi];
-          if (e.weight_mode != kWeightNone) in1[i] = __builtin_nontemporal_load(&gmem(e.weight)[gi]);
+          in1[i] = e.weight_mode != kWeightNone ? __builtin_nontemporal_load(&gmem(e.weight)[gi]) : e.wconst;
         }
       }
     }
@@ -423,14 +422,14 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
         float w = in1[i];
         if (e.weight_mode == kWeightVirtual) {
           const double r = (double)in1[i] / (double)gmem(e.wsum)[gi];
-          w = (float)fmin(1.0, r * e.osem);
+          w = (float)java_min(1.0, r * e.osem);
         }
         const float last = in0[i];
-        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
+        const float next = rl_next_value(last, val, w, e.lambda);
         gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
-        my_max = fmaxf(my_max, change);
+        my_max = java_maxf(my_max, change);
       }
     }
   };
@@ -476,7 +475,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   if (EPI == kEpiUpdate || EPI == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
-      my_max = fmaxf(my_max, __shfl_xor(my_max, o));
+      my_max = java_maxf(my_max, __shfl_xor(my_max, o));
     }
     const int wid = tid / 64;
     if ((tid & 63) == 0) {
@@ -489,7 +488,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       float m = -1.f;
       for (int i = 0; i < kXYThreads / 64; ++i) {
         s += red_sum[i];
-        m = fmaxf(m, red_max[i]);
+        m = java_maxf(m, red_max[i]);
       }
       gmem(e.part_sum)[bx] = s;
       gmem(e.part_max)[bx] = m;

@@ -224,7 +224,12 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
   JNA_HIP(launch_xy_forward_passes(*sp, g, c->block, kGatherZero, c->spec, c->stream));
   JNA_HIP(launch_z(*sp, c->spec, ks->spec, 1.f, 0, bd.z, c->stream));
   JNA_HIP(launch_yx_inverse_passes(*sp, g, c->spec, e, c->stream, nullptr));
-  JNA_HIP(staged_d2h(out, c->block, sizeof(float) * nb, c->stream));
+  // in place (out == im): im is overwritten only once the whole result has
+  // reached host memory, so a failed copy leaves the caller's block as it was
+  if (out == im)
+    JNA_HIP(staged_d2h_all_or_nothing(out, c->block, sizeof(float) * nb, c->stream));
+  else
+    JNA_HIP(staged_d2h(out, c->block, sizeof(float) * nb, c->stream));
   JNA_HIP(hipStreamSynchronize(c->stream));
   return true;
 }

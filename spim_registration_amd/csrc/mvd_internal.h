@@ -204,6 +204,7 @@ struct EpilogueArgs {
   double lambda;
   double* part_sum;      // per-workgroup statistics of the update
   float* part_max;
+  float wconst = 1.f;    // the weight of every voxel under kWeightNone
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -241,6 +242,8 @@ hipError_t launch_sum(const float* a, int64_t n, double* part_sum, int nparts, h
 // they return when the copy is complete.
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t stream);
 hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t stream);
+// Device to host that leaves `dst` untouched unless the whole copy succeeded.
+hipError_t staged_d2h_all_or_nothing(void* dst, const void* src, size_t bytes, hipStream_t stream);
 // Output path (export.hip): min/max with Math.min/max semantics, the 16-bit
 // range widening and the TIFF stack writer (img in device or host memory).
 hipError_t launch_min_max(const float* img, int64_t n, float* minmax, hipStream_t stream);

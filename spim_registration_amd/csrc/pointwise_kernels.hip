@@ -4,6 +4,7 @@
 // reduction writes one partial per workgroup so results do not depend on
 // scheduling.
 #include "mvd_internal.h"
+#include "pointwise_math.h"
 
 namespace mvd {
 
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(kPwThreads) void k_reduce_stats(const double* part_
   float m = -1.f;
   for (int i = threadIdx.x; i < n; i += kPwThreads) {
     s += part_sum[i];
-    m = fmaxf(m, part_max[i]);
+    m = java_maxf(m, part_max[i]);
   }
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = m;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kPwThreads) void k_reduce_stats(const double* part_
   for (int o = kPwThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       rs[threadIdx.x] += rs[threadIdx.x + o];
-      rm[threadIdx.x] = fmaxf(rm[threadIdx.x], rm[threadIdx.x + o]);
+      rm[threadIdx.x] = java_maxf(rm[threadIdx.x], rm[threadIdx.x + o]);
     }
     __syncthreads();
   }
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(kPwThreads) void k_reduce_stats_batch(const ReduceI
   float m = -1.f;
   for (int i = threadIdx.x; i < it.n; i += kPwThreads) {
     s += it.part_sum[i];
-    m = fmaxf(m, it.part_max[i]);
+    m = java_maxf(m, it.part_max[i]);
   }
   rs[threadIdx.x] = s;
   rm[threadIdx.x] = m;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kPwThreads) void k_reduce_stats_batch(const ReduceI
   for (int o = kPwThreads / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       rs[threadIdx.x] += rs[threadIdx.x + o];
-      rm[threadIdx.x] = fmaxf(rm[threadIdx.x], rm[threadIdx.x + o]);
+      rm[threadIdx.x] = java_maxf(rm[threadIdx.x], rm[threadIdx.x + o]);
     }
     __syncthreads();
   }
@@ -231,11 +232,12 @@ hipError_t launch_weight_normalize(float* weights, int nviews, int64_t n, int64_
 }
 
 // ProcessForDeconvolution.adjustForOSEM (deconv/ProcessForDeconvolution.java:372-389):
-// w = min(1, w * (float)osem) in float.
+// w = Math.min(1, w * (float)osem) in float.  A weight that normalisation left
+// NaN (a voxel no view weights, 0/0) stays NaN, as Java's Math.min keeps it.
 __global__ __launch_bounds__(kPwThreads) void k_osem_scale(float* w, int64_t n, float osem) {
 #pragma clang fp contract(off)
   for (int64_t i = blockIdx.x * (int64_t)kPwThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kPwThreads)
-    w[i] = fminf(1.f, w[i] * osem);
+    w[i] = java_minf(1.f, w[i] * osem);
 }
 
 hipError_t launch_osem_scale(float* weights, int64_t n, float osem, hipStream_t stream) {

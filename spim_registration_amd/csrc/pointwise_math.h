@@ -19,19 +19,16 @@ __device__ __forceinline__ int mirror_single(int p, int n) {
   return p < n ? p : period - p;
 }
 
-// MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726).
-// inv_lambda = 1 / lambda, computed once by the caller: the division by lambda
-// becomes a multiplication (the double result may move by one ulp; the float
-// it is rounded to practically never does).
-__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda,
-                                               double inv_lambda) {
+// MVDeconvolution.computeNextValue + tikhonov (deconv/MVDeconvolution.java:692-726),
+// including the division by lambda in double.
+__device__ __forceinline__ float rl_next_value(float last, float integral, float weight, double lambda) {
 #pragma clang fp contract(off)
   const float minv = 0.0001f;
   const float value = last * integral;
   float adjusted;
   if (value > 0.f) {
     if (lambda > 0.0)
-      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) * inv_lambda);
+      adjusted = (float)((sqrt(1.0 + 2.0 * lambda * (double)value) - 1.0) / lambda);
     else
       adjusted = value;
   } else {
@@ -41,12 +38,25 @@ __device__ __forceinline__ float rl_next_value(float last, float integral, float
   return last + ((next - last) * weight);
 }
 
-__device__ __forceinline__ double rl_inv_lambda(double lambda) { return lambda > 0.0 ? 1.0 / lambda : 0.0; }
-
 // Quotient of the observed view by the blurred estimate (MVDeconvolution.computeQuotient, deconv/MVDeconvolution.java:494-546).
 __device__ __forceinline__ float rl_quotient(float blurred, float img) {
 #pragma clang fp contract(off)
   return img > 0.f ? img / blurred : 1.f;
+}
+
+// java.lang.Math.max / min on floats: NaN if either argument is NaN (fmaxf and
+// fminf return the other argument instead).  The per-view change statistics
+// (MVDeconvolution.java:647,677,454) and the OSEM clamp
+// (ProcessForDeconvolution.java:382) keep NaN this way.
+__device__ __forceinline__ float java_maxf(float a, float b) {
+  return a != a ? a : b != b ? b : fmaxf(a, b);
+}
+__device__ __forceinline__ float java_minf(float a, float b) {
+  return a != a ? a : b != b ? b : fminf(a, b);
+}
+// java.lang.Math.min on doubles (NormalizingRandomAccess.get, weights/NormalizingRandomAccess.java:62).
+__device__ __forceinline__ double java_min(double a, double b) {
+  return a != a ? a : b != b ? b : fmin(a, b);
 }
 
 }  // namespace mvd

@@ -446,6 +446,7 @@ struct mvd_session {
   Int3 block{0, 0, 0};
   int weight_mode = -1;  // -1: decided at prepare
   double osem = 1.0;
+  float wconst = 1.f;  // weight of every voxel without weight images
   bool prepared = false, psi_ready = false;
 
   ~mvd_session() {
@@ -614,6 +615,7 @@ void conv_pass_descs(mvd_session* s, int v, int conv, const std::vector<float2*>
       xi.e.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
       xi.e.wsum = s->d_wsum.as<float>();
       xi.e.osem = s->osem;
+      xi.e.wconst = s->wconst;
       xi.e.lambda = s->lambda;
       xi.e.part_sum = s->d_part_sum.as<double>() + off;
       xi.e.part_max = s->d_part_max.as<float>() + off;
@@ -1174,7 +1176,9 @@ int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value,
   for (auto& v : s->views) all_w = all_w && v.has_weight;
   MVD_REQUIRE(mode == MVD_WEIGHTS_NONE || all_w, "this weight mode needs a weight image for every view");
   DeviceGuard guard(s->device);
-  double mn = 1, avg = 1;
+  // Without weights there is no WeightNormalizer and the overlap counts keep
+  // their initial 0 (ProcessForDeconvolution.java:89-90,331-351).
+  double mn = mode == MVD_WEIGHTS_NONE ? 0 : 1, avg = mode == MVD_WEIGHTS_NONE ? 0 : 1;
   if (mode == MVD_WEIGHTS_PRECOMPUTED || mode == MVD_WEIGHTS_VIRTUAL) {
     // WeightNormalizer.process (deconv/WeightNormalizer.java:73-130): per portion
     // of 2*threads, min count and average count; the overall average is the mean
@@ -1223,6 +1227,10 @@ int mvd_set_weights(mvd_session* s, int mode, int osem_index, double osem_value,
   if (osem != 1.0 && (mode == MVD_WEIGHTS_PRECOMPUTED || mode == MVD_WEIGHTS_AS_GIVEN))
     MVD_CHECK(launch_osem_scale(s->d_weights.as<float>(), s->n * s->nviews, (float)osem, s->stream));
   MVD_CHECK(hipStreamSynchronize(s->stream));
+  // NO_WEIGHTS: the constant weight image becomes Math.min(1, 1 * (float)osem)
+  // (ProcessForDeconvolution.java:385-394); osem is never NaN here.
+  s->wconst = 1.f;
+  if (osem != 1.0 && mode == MVD_WEIGHTS_NONE) s->wconst = std::min(1.f, 1.f * (float)osem);
   s->osem = osem;
   s->weight_mode = mode;
   if (min_avg_out) {
@@ -1407,6 +1415,7 @@ int mvd_iterate(mvd_session* s, int n, double* stats) {
       u.weight = wm == kWeightNone ? nullptr : s->d_weights.as<float>() + v * s->n;
       u.wsum = s->d_wsum.as<float>();
       u.osem = s->osem;
+      u.wconst = s->wconst;
       u.lambda = s->lambda;
       int off = 0;
       for (auto& g : view.blocks) {

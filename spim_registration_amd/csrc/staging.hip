@@ -8,6 +8,8 @@
 // The two 16 MiB pinned buffers of a device are allocated at its first large
 // copy and held until the process ends; large copies on one device are
 // serialised by a mutex (the JNA hooks are called from one thread per device).
+// The all-or-nothing variant keeps one more pinned buffer per device, as large
+// as the largest such copy so far, also held until the process ends.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -28,6 +30,8 @@ struct Stage {
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool ready = false;
+  void* whole = nullptr;  // pinned buffer of the all-or-nothing copies (grows, held)
+  size_t whole_cap = 0;
 };
 Stage g_stage[kMaxDevices];
 
@@ -137,6 +141,29 @@ hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t stre
   }
   const hipError_t es = hipStreamSynchronize(stream);
   return e != hipSuccess ? e : es;
+}
+
+// Device to host where the host array must stay as it was unless the whole
+// copy succeeds: the DMA fills one pinned buffer of the full size, and only
+// after it has completed without error is the host array overwritten.
+hipError_t staged_d2h_all_or_nothing(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  Stage& s = g_stage[dev];
+  std::lock_guard<std::mutex> lk(s.mu);
+  if (s.whole_cap < bytes) {
+    if (s.whole) (void)hipHostFree(s.whole);
+    s.whole = nullptr;
+    s.whole_cap = 0;
+    if ((e = hipHostMalloc(&s.whole, bytes, hipHostMallocDefault)) != hipSuccess) return e;
+    s.whole_cap = bytes;
+  }
+  if ((e = hipMemcpyAsync(s.whole, src, bytes, hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  parallel_copy(dst, s.whole, bytes);
+  return hipSuccess;
 }
 
 }  // namespace mvd

@@ -253,3 +253,43 @@ def test_mask_zero_where_no_data():
     v1[:, :, :3] = 0
     psi, _ = O.deconvolve([v0, v1], ds.psfs, O.INDEPENDENT, 2, 0.0, norm_threads=2)
     assert np.all(psi[:, :, :3] == 0) and np.all(psi[:, :, 3:] > 0)
+
+
+def test_overlap_counts_closed_form():
+    """4 voxels with 0, 1, 2, 2 weighted views, numThreads 1: portions of 2
+    voxels give minima 0 and 2 and averages 0.5 and 2."""
+    a = np.array([0, 1, 1, 1], np.float32).reshape(1, 1, 4)
+    b = np.array([0, 0, 2, 0.5], np.float32).reshape(1, 1, 4)
+    mn, avg = O.overlap_counts([a, b], 1)
+    assert mn == 1  # max(1, 0)
+    assert avg == 1.25
+
+
+def test_effective_weights_closed_forms():
+    rng = np.random.default_rng(2)
+    raw = [rng.random((3, 4, 5)).astype(np.float32) for _ in range(3)]
+    for r in raw:
+        r[0, 0, :] = 0  # a row no view weights
+    raw[1][1] = 0
+    with np.errstate(invalid="ignore"):
+        ws, osem = O.effective_weights(raw, O.W_PRECOMPUTED, 0)
+    assert osem == 1.0
+    total = sum(w.astype(np.float64) for w in ws)
+    assert np.all(np.isnan(total[0, 0, :]))  # 0/0, no guard
+    np.testing.assert_allclose(total[1:], 1.0, rtol=1e-6)
+    # OSEM keeps NaN (Math.min) and clamps at 1
+    ws2, osem2 = O.effective_weights(raw, O.W_PRECOMPUTED, 3, 2.5)
+    assert osem2 == 2.5
+    assert np.all(np.isnan(ws2[0][0, 0, :]))
+    fin = ~np.isnan(ws[0])
+    np.testing.assert_array_equal(ws2[0][fin], np.minimum(np.float32(1), ws[0][fin] * np.float32(2.5)))
+    # virtual weights: w / max(sum, 1) * osem, clamped at 1, never NaN
+    wv, _ = O.effective_weights(raw, O.W_VIRTUAL, 3, 1.5)
+    s = sum(r.astype(np.float64) for r in raw)
+    expect = np.minimum(1.0, raw[2].astype(np.float64) / np.where(s > 1, s.astype(np.float32), 1) * 1.5)
+    np.testing.assert_array_equal(wv[2], expect.astype(np.float32))
+    assert not np.isnan(wv[2]).any() and wv[2].max() <= 1
+    # no weights: the overlap counts stay 0, so OSEM index 1 and 2 give weight 0
+    for idx, val, c in ((0, 7.0, 1.0), (1, 7.0, 0.0), (2, 7.0, 0.0), (3, 0.25, 0.25), (3, 3.0, 1.0)):
+        wn, _ = O.effective_weights(raw, O.W_NONE, idx, val, shape=(2, 2, 2))
+        assert np.all(wn[0] == np.float32(c))
