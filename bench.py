@@ -91,17 +91,30 @@ def block_effective_sizes(img, ker, bs):
     return effs
 
 
-def cpu_baseline(ds, w, threads):
-    """Oracle float32 path: one view-iteration of the workload (all its blocks)."""
+def host_fields(threads):
+    """Host description recorded with every CPU baseline."""
+    from oracle import oracle as O
+    return dict(nproc=O.nproc(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"), threads=threads,
+                cpu_model=O.cpu_model())
+
+
+def cpu_baseline(ds, w, threads, repeats=3):
+    """Oracle float32 path: one view-iteration of the workload (all its blocks),
+    median of `repeats` samples; each sample is a 1-iteration run minus a
+    0-iteration run (kernel preparation and FFT set-up excluded)."""
     from oracle import oracle as O
     views, psfs = ds.views[:1], ds.psfs[:1]
     blk = w["block"]
-    t0 = time.perf_counter()
-    O.deconvolve(views, psfs, O.EFFICIENT_BAYESIAN, 0, w["lam"], block_xyz=blk, norm_threads=8, threads=threads)
-    t1 = time.perf_counter()
-    O.deconvolve(views, psfs, O.EFFICIENT_BAYESIAN, 1, w["lam"], block_xyz=blk, norm_threads=8, threads=threads)
-    t2 = time.perf_counter()
-    it = (t2 - t1) - (t1 - t0)
+
+    def run(iters):
+        t0 = time.perf_counter()
+        O.deconvolve(views, psfs, O.EFFICIENT_BAYESIAN, iters, w["lam"], block_xyz=blk, norm_threads=8,
+                     threads=threads)
+        return time.perf_counter() - t0
+
+    setup = run(0)
+    samples = sorted(run(1) - setup for _ in range(repeats))
+    it = samples[len(samples) // 2]
     n = int(np.prod(w["size_xyz"]))
     sx, sy, sz = w["size_xyz"]
     if blk:
@@ -111,7 +124,9 @@ def cpu_baseline(ds, w, threads):
         where = "one window of img + K - 1"
     return dict(value=n / it / 1e9, unit=UNIT, cores=threads, kind="port",
                 sample=f"oracle float32 CPU path, 1 view x 1 iteration of the {sx}x{sy}x{sz} workload "
-                       f"({where}), {it:.2f} s, setup excluded")
+                       f"({where}), median of {repeats}: {it:.2f} s (samples "
+                       f"{', '.join(f'{x:.2f}' for x in samples)} s), setup excluded",
+                **host_fields(threads))
 
 
 def run_c3(args):
@@ -383,7 +398,8 @@ def run_resample(args):
     exact = np.array_equal(outs[1].cpu().numpy(), img) and np.array_equal(wts[1].cpu().numpy(), w)
     cpu = None
     if not args.no_cpu_baseline:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        from oracle.oracle import default_threads
+        threads = default_threads()
         t0 = time.perf_counter()
         O.transform_view(raw1, invs[1], (0, 0, 0), box, blend.border, blend.range, threads=threads)
         sec = time.perf_counter() - t0
@@ -763,9 +779,10 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        from oracle.oracle import default_threads
+        threads = default_threads()
         t = time.perf_counter()
-        cpu = cpu_baseline(ds, w, threads)
+        cpu = cpu_baseline(ds, w, threads, repeats=1 if args.workload == "c5" else 3)
         log(f"cpu baseline {cpu['value']:.4f} {UNIT} ({time.perf_counter() - t:.1f} s)")
     d.close()
 

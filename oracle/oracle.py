@@ -96,8 +96,34 @@ def _ptr_array(arrays):
     return (_P * len(arrays))(*[a.ctypes.data for a in arrays])
 
 
+def nproc() -> int:
+    """CPUs this process may run on (what `nproc` prints)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def default_threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """nproc, capped by OMP_NUM_THREADS when the host sets one (a share of a
+    larger machine)."""
+    n = nproc()
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def divide_portions(n: int, k: int):
