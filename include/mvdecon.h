@@ -74,6 +74,23 @@ void convolution3DfftCUDAInPlace(float* im, int* imDim, float* kernel, int* kern
 float* convolution3DfftCUDA(float* im, int* imDim, float* kernel, int* kernelDim, int devCUDA);
 void mvd_free(void* p);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the block convolution of convolution3DfftCUDAInPlace on
+ * device arrays, allocating nothing.  im: device block of imDim = {nz, ny,
+ * nx} floats, overwritten with the result; kernel: device array of
+ * kernelDim = {kz, ky, kx} floats; dims are host arrays, as in the JNA call.
+ * Enqueued on `stream` (a hipStream_t, or NULL) and asynchronous: the result
+ * is in im when the stream reaches it.  The kernel spectrum is recomputed on
+ * every call.  Limit: the FFT length of each axis -- the smallest 5-smooth
+ * size >= the block size, even along x -- at most 128 (blocks up to 128^3);
+ * larger blocks are rejected with an error.  Scratch: one ~20 MiB device
+ * array per GPU, held until the process ends, shared by every call on that
+ * GPU: calls that overlap on one GPU race.  Returns 0, or -1 (invalid
+ * arguments) / -2 (device error) with mvd_last_error() set. */
+int mvd_convolution3d_inplace_noalloc(float* im, const int* imDim, const float* kernel, const int* kernelDim,
+                                      int devCUDA, void* stream);
+
 /* Replace CUDASeparableConvolution.convolve_{7,15,31,63,127}
  * (spim/process/cuda/CUDASeparableConvolution.java:34-38), the second native
  * hook (difference-of-Gaussian detection, via

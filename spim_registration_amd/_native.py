@@ -18,7 +18,7 @@ _lib = None
 EXPORTS = (
     "getNumDevicesCUDA", "getNameDeviceCUDA", "getMemDeviceCUDA", "getFreeMemDeviceCUDA",
     "getCUDAcomputeCapabilityMajorVersion", "getCUDAcomputeCapabilityMinorVersion",
-    "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free",
+    "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "mvd_free", "mvd_convolution3d_inplace_noalloc",
     "convolve_7", "convolve_15", "convolve_31", "convolve_63", "convolve_127", "convolutionCPU",
     "mvd_create", "mvd_destroy", "mvd_add_view", "mvd_add_view_transformed", "mvd_transform_view",
     "mvd_transform_view_device", "mvd_extract_psf", "mvd_psf_transform_size",
@@ -58,6 +58,8 @@ def _declare(L):
     L.convolutionCPU.restype = None
     L.mvd_free.argtypes = [_P]
     L.mvd_free.restype = None
+    L.mvd_convolution3d_inplace_noalloc.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P]
+    L.mvd_convolution3d_inplace_noalloc.restype = ctypes.c_int
     L.mvd_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]
     L.mvd_destroy.argtypes = [_P]
     L.mvd_add_view.argtypes = [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int]

@@ -71,13 +71,18 @@ DeviceContext* context(int dev) {
   return g_ctx[dev].get();
 }
 
-std::vector<float2> twiddle_table(int n) {
-  std::vector<float2> t(n);
+// exp(-2 pi i k / n), k < n, computed in long double and rounded to float.
+void fill_twiddles(float2* t, int n) {
   const long double two_pi = 6.283185307179586476925286766559L;
   for (int k = 0; k < n; ++k) {
     const long double a = -two_pi * (long double)k / (long double)n;
     t[k] = make_float2((float)cosl(a), (float)sinl(a));
   }
+}
+
+std::vector<float2> twiddle_table(int n) {
+  std::vector<float2> t(n);
+  fill_twiddles(t.data(), n);
   return t;
 }
 
@@ -236,6 +241,54 @@ bool convolve_block(const float* im, Int3 bd, const float* kernel, Int3 kd, int 
 
 bool dims_ok(const int* d) { return d && d[0] > 0 && d[1] > 0 && d[2] > 0; }
 
+// ---------------------------------------------------------------------------
+// The no-allocation variant: FFT lengths up to kNaMaxLen per axis; twiddles,
+// block spectrum and kernel spectrum live in the file-scope array `scratch`.
+constexpr int kNaMaxLen = 128;
+constexpr int64_t kNaTwElems = 4 * kNaMaxLen;  // tw_x | tw_y | tw_z | unused (keeps 4 KB alignment)
+constexpr int64_t kNaSpecElems = (int64_t)((kNaMaxLen / 2 + 1 + 15) / 16 * 16) * kNaMaxLen * kNaMaxLen;
+constexpr int64_t kNaElems = kNaTwElems + 2 * kNaSpecElems;
+
+// Twiddle tables handed over by value in the kernel arguments (3 KB), so that
+// filling them needs no host-to-device copy.
+struct NaTwiddles {
+  float2 t[3 * kNaMaxLen];
+};
+static_assert(sizeof(NaTwiddles) <= 3200, "kernel argument size");
+
+__global__ void k_na_twiddles(float2* dst, NaTwiddles tw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * kNaMaxLen) dst[i] = tw.t[i];
+}
+
+}  // namespace
+}  // namespace mvd
+
+// Scratch of mvd_convolution3d_inplace_noalloc, one copy per device, held
+// until the process ends (about 20 MiB).  Host code never uses this name as
+// an address: it takes each device's address from hipGetSymbolAddress.
+static __device__ __align__(256) float2 scratch[mvd::kNaElems];
+
+namespace mvd {
+namespace {
+
+std::mutex g_na_mu;
+float2* g_na_ptr[64] = {};
+
+// Device address of `scratch` on the current device `dev`, looked up once.
+hipError_t na_scratch(int dev, float2** out) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_na_ptr[dev] = static_cast<float2*>(p);
+  }
+  *out = g_na_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 }  // namespace mvd
 
@@ -336,5 +389,80 @@ float* convolution3DfftCUDA(float* im, int* imDim, float* kernel, int* kernelDim
 }
 
 void mvd_free(void* p) { std::free(p); }
+
+int mvd_convolution3d_inplace_noalloc(float* im, const int* imDim, const float* kernel, const int* kernelDim,
+                                      int devCUDA, void* stream) {
+  if (!im || !kernel) return set_error("NULL array"), -1;
+  if (!dims_ok(imDim) || !dims_ok(kernelDim)) return set_error("invalid dimensions"), -1;
+  const Int3 bd = {imDim[2], imDim[1], imDim[0]}, kd = {kernelDim[2], kernelDim[1], kernelDim[0]};
+  const Int3 q = {next_fft_size(bd.x, true), next_fft_size(bd.y, false), next_fft_size(bd.z, false)};
+  if (q.x > kNaMaxLen || q.y > kNaMaxLen || q.z > kNaMaxLen)
+    return set_error("block too large for the no-allocation entry: FFT lengths above 128 per axis"), -1;
+  if (kd.x > q.x || kd.y > q.y || kd.z > q.z) return set_error("kernel larger than the block"), -1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || devCUDA < 0 || devCUDA >= ndev) {
+    (void)hipGetLastError();
+    return set_error("invalid device id"), -1;
+  }
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  struct Restore {
+    int d;
+    ~Restore() {
+      if (d >= 0) (void)hipSetDevice(d);
+    }
+  } restore{prev};
+  hipError_t e = hipSetDevice(devCUDA);
+  float2* base = nullptr;
+  if (e == hipSuccess) e = na_scratch(devCUDA, &base);
+  if (e != hipSuccess) return set_error(std::string("no-allocation scratch: ") + hipGetErrorString(e)), -2;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  ShapePlan sp{};
+  sp.q = q;
+  sp.hx = q.x / 2 + 1;
+  sp.pitch = spectrum_pitch(sp.hx);
+  sp.px = make_plan(q.x / 2);
+  sp.py = make_plan(q.y);
+  sp.pz = make_plan(q.z);
+  sp.bx = lines(q.x / 2 + 1);
+  sp.by = lines(q.y);
+  sp.bz = lines(q.z);
+  sp.tw_x = base;
+  sp.tw_y = base + kNaMaxLen;
+  sp.tw_z = base + 2 * kNaMaxLen;
+  float2* spec = base + kNaTwElems;
+  float2* kspec = spec + kNaSpecElems;
+  if (sp.spectrum_elems() > kNaSpecElems) return set_error("internal: no-allocation spectrum size"), -2;
+  NaTwiddles tw{};
+  fill_twiddles(tw.t, q.x);
+  fill_twiddles(tw.t + kNaMaxLen, q.y);
+  fill_twiddles(tw.t + 2 * kNaMaxLen, q.z);
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_na_twiddles, dim3((3 * kNaMaxLen + 255) / 256), dim3(256), 0, st, base, tw);
+  e = hipGetLastError();
+  // kernel spectrum (wrapped kernel, scaled by 1 / (qx qy qz)), then the block
+  ConvGeom kg{};
+  kg.src = kd;
+  kg.win = q;
+  kg.q = q;
+  if (e == hipSuccess) e = launch_x_forward(sp, kg, kernel, kGatherKernelWrap, kspec, st);
+  if (e == hipSuccess) e = launch_y(sp, kspec, 0, q.z, 0, q.y, -1.f, st);
+  const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
+  if (e == hipSuccess) e = launch_z(sp, kspec, nullptr, scale, 0, q.z, st);
+  ConvGeom g{};
+  g.src = bd;
+  g.win = bd;
+  g.q = q;
+  g.eff_size = bd;
+  g.dst = bd;
+  EpilogueArgs ep{};
+  ep.mode = kEpiStore;
+  ep.dst = im;
+  if (e == hipSuccess) e = launch_xy_forward_passes(sp, g, im, kGatherZero, spec, st);
+  if (e == hipSuccess) e = launch_z(sp, spec, kspec, 1.f, 0, bd.z, st);
+  if (e == hipSuccess) e = launch_yx_inverse_passes(sp, g, spec, ep, st, nullptr);
+  if (e != hipSuccess) return set_error(std::string("no-allocation convolution: ") + hipGetErrorString(e)), -2;
+  return 0;
+}
 
 }  // extern "C"
