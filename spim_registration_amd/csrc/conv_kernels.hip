@@ -254,6 +254,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
   lds_fft(buf, lp, B, P.plan, tw, 2, 1.f);
   const float* fbuf = reinterpret_cast<const float*>(buf);
   const EpilogueArgs& e = P.e;
+  const double inv_lambda = rl_inv_lambda(e.lambda);
   double my_sum = 0.0;
   float my_max = -1.f;
   const int nx = g.eff_size.x;
@@ -288,7 +289,7 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
         w = (float)java_min(1.0, r * e.osem);
       }
       const float last = gmem(e.dst)[gi];
-      const float next = rl_next_value(last, v, w, e.lambda);
+      const float next = rl_next_value(last, v, w, e.lambda, inv_lambda);
       gmem(e.dst)[gi] = next;
       const float change = fabsf(next - last);
       my_sum += (double)change;

@@ -507,6 +507,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   __syncthreads();
   const float* fbuf = reinterpret_cast<const float*>(buf);
   const EpilogueArgs& e = P.e;
+  const double inv_lambda = rl_inv_lambda(e.lambda);
   double my_sum = 0.0;
   float my_max = -1.f;
   const int nx = g.eff_size.x;
@@ -570,7 +571,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
           w = (float)java_min(1.0, rr * e.osem);
         }
         const float last = in0[r][ee];
-        const float next = rl_next_value(last, val, w, e.lambda);
+        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
         gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;

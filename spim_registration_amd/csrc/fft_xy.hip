@@ -19,6 +19,7 @@
 // the twiddle table -- about 137 KB, one workgroup per CU.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "fft_regs.h"
 #include "mvd_internal.h"
@@ -35,7 +36,6 @@ constexpr int kXYMP = kXYM + 1;  // LDS row pitch of an x half (complex)
 constexpr int kXYLP = 65;        // LDS pitch of a y round: 64 columns + 1
 constexpr int kXYHalf = 128;     // rows per x half
 constexpr int kXYLds = kXYN * kXYLP;
-constexpr int kXYEpiRows = 8;    // epilogue rows whose loads are issued together
 static_assert(kXYHalf * kXYMP <= kXYLds, "x half must fit the y-round buffer");
 
 using SX = Spec<kXYM>;  // 16 x 8: 16 values per thread, 8 threads per row
@@ -143,8 +143,19 @@ struct XYFwdParams {
   int pitch;
 };
 
+// Where the forward pass stores a plane's spectrum: the same plane can be
+// needed by several block windows (see PlaneTask), so up to kMaxPlaneDests
+// copies are written.
+struct PlaneOut {
+  int n;
+  const PlaneTask* task;     // destinations of a task (n of them), or
+  const void* items;         // (the launch's XYFwdParams)
+  float2* single;            // the one destination without a task
+  __device__ float2* base(int d, int pitch) const;  // start of the plane in destination d
+};
+
 template <int MODE>
-__device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
+__device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx, const PlaneOut& po) {
   __shared__ float2 lds[kXYLds];
   __shared__ float2 dcn[kXYN];
   __shared__ float2 tw[kXYN];
@@ -227,7 +238,6 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     }
   }
   // y transforms, 64 columns per round
-  const auto out = gmem(P.spec) + (int64_t)z * kXYN * P.pitch;
   auto yaddr = [cl](int i) { return i * kXYLP + cl; };
   xy_fft<kXYN>(c0, lds, jy, tw, 1, -1.f, yaddr);
   // Column 0 carries Z = F0 + i F128, F0 and F128 the transforms of the real
@@ -249,30 +259,50 @@ __device__ __forceinline__ void xy_forward_body(const XYFwdParams& P, int bx) {
     const int kk = elem_index<kXYN, 16, SY::R1>(jy, 0, cl);
     sdc = cscale(cadd(dcn[kk], cconj(dcn[(kXYN - kk) & (kXYN - 1)])), 0.5f);
   }
+  // (lane indices from opaque() in every destination pass: offsets hoisted
+  // out of the destination loop would hold 32 more registers)
+  for (int d = 0; d < po.n; ++d) {
+    const auto out = gmem(po.base(d, P.pitch));
+    const int lane = opaque(threadIdx.x), cl_ = lane & 63, jy_ = lane >> 6;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    const float2 d0 = make_float2(__shfl(sdc.x, t), __shfl(sdc.y, t));
-    stream_store(&out[ky * P.pitch + cl], cl == 0 ? d0 : c0[t]);
+    for (int t = 0; t < 16; ++t) {
+      const int ky = elem_index<kXYN, 16, SY::R1>(jy_, 0, t);
+      const float2 d0 = make_float2(__shfl(sdc.x, t), __shfl(sdc.y, t));
+      stream_store(&out[ky * P.pitch + cl_], cl_ == 0 ? d0 : c0[t]);
+    }
   }
   xy_fft<kXYN>(c1, lds, jy, tw, 1, -1.f, yaddr);
+  for (int d = 0; d < po.n; ++d) {
+    const auto out = gmem(po.base(d, P.pitch));
+    const int lane = opaque(threadIdx.x), cl_ = lane & 63, jy_ = lane >> 6;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    stream_store(&out[ky * P.pitch + 64 + cl], c1[t]);
+    for (int t = 0; t < 16; ++t) {
+      const int ky = elem_index<kXYN, 16, SY::R1>(jy_, 0, t);
+      stream_store(&out[ky * P.pitch + 64 + cl_], c1[t]);
+    }
   }
   // the Nyquist column and the padding of this wave's 16 rows: 16 lanes per
   // row, four rows per store instruction (pitch - 128 = 16 columns)
+  float2 nq[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int t = 4 * q + (cl >> 4), c = cl & 15;
     const int ky = elem_index<kXYN, 16, SY::R1>(jy, 0, t);
-    float2 v = make_float2(0.f, 0.f);
+    nq[q] = make_float2(0.f, 0.f);
     if (c == 0) {
       const float2 d = cscale(csub(dcn[ky], cconj(dcn[(kXYN - ky) & (kXYN - 1)])), 0.5f);
-      v = make_float2(d.y, -d.x);
+      nq[q] = make_float2(d.y, -d.x);
     }
-    stream_store(&out[ky * P.pitch + kXYM + c], v);
+  }
+  for (int d = 0; d < po.n; ++d) {
+    const auto out = gmem(po.base(d, P.pitch));
+    const int lane = opaque(threadIdx.x), cl_ = lane & 63, jy_ = lane >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = 4 * q + (cl_ >> 4), c = cl_ & 15;
+      const int ky = elem_index<kXYN, 16, SY::R1>(jy_, 0, t);
+      stream_store(&out[ky * P.pitch + kXYM + c], nq[q]);
+    }
   }
 }
 
@@ -294,6 +324,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int tid = threadIdx.x;
   const ConvGeom& g = P.g;
   const EpilogueArgs& e = P.e;
+  const double inv_lambda = rl_inv_lambda(e.lambda);
   const int zo = bx;                 // output plane inside the effective box
   const int z = g.eff_local.z + zo;  // plane of the FFT box
   if (tid < kXYN) tw[tid] = gmem(P.tw)[tid];
@@ -365,22 +396,26 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   const int ylo = g.eff_local.y, yhi = g.eff_local.y + g.eff_size.y;
   const int nx = g.eff_size.x;
   const float* fbuf = reinterpret_cast<const float*>(lds);
-  constexpr int R = kXYEpiRows;
   // images hold fewer than 2^31 voxels (checked by the host): 32-bit offsets
   auto out_index = [&](int x, int y, int zo_) -> int {
     return (g.eff_offset.x + x) + g.dst.x * ((g.eff_offset.y + y - ylo) + g.dst.y * (g.eff_offset.z + zo_));
   };
-  // inputs of the epilogue for rows rsub + 4 (c R + i) of half h; lanes run along x
+  // A thread's epilogue voxels in half h are x = lane & 255 and the 32 rows
+  // rsub + 4 j, j < 32.  Their inputs are loaded in groups of R rows
+  // (in0: the view or the estimate, in1: the weight), as early as registers
+  // allow: the waves' loads are then in flight during the x transform, not
+  // waited for one small batch after another.
   // The views and weights are read once per iteration: streaming loads keep
   // them from displacing the estimate and ratio lines that the forward gathers
   // of neighbouring blocks (overlapping windows) read again.
-  auto load_batch = [&](int h, int c, float (&in0)[R], float (&in1)[R]) {
+  auto load_rows = [&](int h, int j0, auto& in0, auto& in1) {
+    constexpr int R = sizeof(in0) / sizeof(float);
     const int lane = opaque(threadIdx.x);
     const int x = lane & 255, rsub = lane >> 8;
     const bool x_live = x < nx;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-      const int y = h * kXYHalf + rsub + 4 * (c * R + i);
+      const int y = h * kXYHalf + rsub + 4 * (j0 + i);
       in0[i] = 0.f;
       in1[i] = 1.f;
       if (x_live && y >= ylo && y < yhi) {
@@ -395,13 +430,14 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       }
     }
   };
-  auto run_batch = [&](int h, int c, const float (&in0)[R], const float (&in1)[R]) {
+  auto run_rows = [&](int h, int j0, const auto& in0, const auto& in1) {
+    constexpr int R = sizeof(in0) / sizeof(float);
     const int lane = opaque(threadIdx.x);
     const int x = lane & 255, rsub = lane >> 8;
     const bool x_live = x < nx;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-      const int yl = rsub + 4 * (c * R + i);
+      const int yl = rsub + 4 * (j0 + i);
       const int y = h * kXYHalf + yl;
       if (!x_live || y < ylo || y >= yhi) continue;
       const int gi = out_index(x, y, zo);
@@ -425,7 +461,7 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
           w = (float)java_min(1.0, r * e.osem);
         }
         const float last = in0[i];
-        const float next = rl_next_value(last, val, w, e.lambda);
+        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
         gmem(e.dst)[gi] = next;
         const float change = fabsf(next - last);
         my_sum += (double)change;
@@ -433,22 +469,8 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       }
     }
   };
-  float p0[R], p1[R];  // first batch of a half, loaded ahead of the transform where registers allow
-  // the first half's batch can be in flight during its transform only for the
-  // one-input epilogues: with two inputs the held registers would spill
-  constexpr bool kEarlyLoad = EPI == kEpiQuotient || EPI == kEpiStore;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    if (h == 1) {
-      __syncthreads();  // the first half's readers of lds are done
-      put_rows(c0, 0, 1);
-      put_rows(c1, 64, 1);
-      load_batch(1, 0, p0, p1);  // no plane values are held in registers any more
-    } else if constexpr (kEarlyLoad) {
-      load_batch(0, 0, p0, p1);  // in flight during the first half's transform
-    }
-    __syncthreads();
-    // pack X[k], X[M - k] into Z[k] = E + i O
+  // pack X[k], X[M - k] into Z[k] = E + i O, then the 128-point row transforms
+  auto rows_c2r = [&]() {
     for (int idx = opaque(threadIdx.x); idx < kXYHalf * (kXYM / 2 + 1); idx += kXYThreads) {
       const int l = idx / (kXYM / 2 + 1), k = idx % (kXYM / 2 + 1);
       float2* row = lds + l * kXYMP;
@@ -462,16 +484,39 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
     __syncthreads();
     xy_rows_fft(lds, tw, 1.f);
     __syncthreads();
-    // epilogue over this half's output rows; lanes run along x
-    if (!kEarlyLoad && h == 0) load_batch(0, 0, p0, p1);
-    run_batch(h, 0, p0, p1);
+  };
+  // The 32 rows of a half in chunks of C: the first chunk's loads are in
+  // flight during the half's transform, and each later chunk is loaded while
+  // the one before it is computed and stored.  The first half still holds the
+  // second half's plane values in registers, the second half holds none;
+  // epilogues with two inputs per voxel take chunks half as long.
+  constexpr int kInputs = (EPI == kEpiUpdate || EPI == kEpiResidual) ? 2 : 1;
+  auto half_epilogue = [&](int h, auto chunk) {
+    constexpr int C = decltype(chunk)::value;
+    float cur0[C], cur1[C];
+    load_rows(h, 0, cur0, cur1);
+    __syncthreads();
+    rows_c2r();
 #pragma unroll
-    for (int c = 1; c < 32 / R; ++c) {
-      float a0[R], a1[R];
-      load_batch(h, c, a0, a1);
-      run_batch(h, c, a0, a1);
+    for (int j0 = 0; j0 < 32; j0 += C) {
+      float nxt0[C], nxt1[C];
+      if (j0 + C < 32) load_rows(h, j0 + C, nxt0, nxt1);
+      run_rows(h, j0, cur0, cur1);
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        cur0[i] = nxt0[i];
+        cur1[i] = nxt1[i];
+      }
     }
-  }
+  };
+  // (chunk sizes: the largest that compile without register spills)
+  constexpr int kChunk0 = EPI == kEpiResidual ? 2 : kInputs == 2 ? 4 : 8;
+  constexpr int kChunk1 = EPI == kEpiResidual ? 16 : kInputs == 2 ? 16 : 32;
+  half_epilogue(0, std::integral_constant<int, kChunk0>{});
+  __syncthreads();  // the first half's readers of lds are done
+  put_rows(c0, 0, 1);
+  put_rows(c1, 64, 1);
+  half_epilogue(1, std::integral_constant<int, kChunk1>{});
   if (EPI == kEpiUpdate || EPI == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
@@ -496,17 +541,41 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
   }
 }
 
+__device__ float2* PlaneOut::base(int d, int pitch) const {
+  if (!task) return single;
+  const XYFwdParams* it = static_cast<const XYFwdParams*>(items);
+  return it[task->dest[d][0]].spec + (int64_t)task->dest[d][1] * kXYN * pitch;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kXYThreads) void k_xy_forward(XYFwdParams P) {
-  xy_forward_body<MODE>(P, blockIdx.x);
+  PlaneOut po{};
+  po.n = 1;
+  po.single = P.spec + (int64_t)blockIdx.x * kXYN * P.pitch;
+  xy_forward_body<MODE>(P, blockIdx.x, po);
 }
+// tasks (optional): per workgroup the plane it transforms and the spectra it
+// stores the result into, in the order the host chose; without tasks,
+// workgroup -> (item, plane) through the prefix array and one store each.
 template <int MODE>
 __global__ __launch_bounds__(kXYThreads) void k_xy_forward_batch(const XYFwdParams* items, const int* prefix,
-                                                                const int* gx, int nitems) {
-  int item, bx, by;
-  batch_locate(prefix, gx, nitems, item, bx, by);
-  const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
-  xy_forward_body<MODE>(item_params, bx);
+                                                                const int* gx, int nitems, const PlaneTask* tasks) {
+  PlaneOut po{};
+  if (tasks) {
+    const PlaneTask* T = tasks + blockIdx.x;
+    const auto item_params = items[T->item];  // private copy: the kernel's stores cannot alias it
+    po.n = T->ndest;
+    po.task = T;
+    po.items = items;
+    xy_forward_body<MODE>(item_params, T->plane, po);
+  } else {
+    int item, bx, by;
+    batch_locate(prefix, gx, nitems, item, bx, by);
+    const auto item_params = items[item];
+    po.n = 1;
+    po.single = item_params.spec + (int64_t)bx * kXYN * item_params.pitch;
+    xy_forward_body<MODE>(item_params, bx, po);
+  }
 }
 template <int EPI>
 __global__ __launch_bounds__(kXYThreads) void k_xy_inverse(XYInvParams P) {
@@ -590,7 +659,7 @@ int build_xy_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) {
 }
 
 hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
-                           int nwg, hipStream_t stream) {
+                           int nwg, hipStream_t stream, const PlaneTask* tasks) {
   if (nwg <= 0) return hipSuccess;
   (void)hipGetLastError();
   const dim3 grid(nwg), block(kXYThreads);
@@ -598,13 +667,13 @@ hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* pref
     const XYFwdParams* it = static_cast<const XYFwdParams*>(items);
     switch (cls) {
       case kGatherMirror:
-        hipLaunchKernelGGL(k_xy_forward_batch<kGatherMirror>, grid, block, 0, stream, it, prefix, gx, nitems);
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherMirror>, grid, block, 0, stream, it, prefix, gx, nitems, tasks);
         break;
       case kGatherOne:
-        hipLaunchKernelGGL(k_xy_forward_batch<kGatherOne>, grid, block, 0, stream, it, prefix, gx, nitems);
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherOne>, grid, block, 0, stream, it, prefix, gx, nitems, tasks);
         break;
       case kGatherZero:
-        hipLaunchKernelGGL(k_xy_forward_batch<kGatherZero>, grid, block, 0, stream, it, prefix, gx, nitems);
+        hipLaunchKernelGGL(k_xy_forward_batch<kGatherZero>, grid, block, 0, stream, it, prefix, gx, nitems, tasks);
         break;
       default: return hipErrorInvalidValue;
     }

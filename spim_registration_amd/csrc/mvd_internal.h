@@ -305,10 +305,22 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
                              int nwg, hipStream_t stream);
 // z passes of many blocks sharing one kernel spectrum and line grid (gx, gy).
 hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream);
+// One workgroup of a fused forward launch with a task list: transform plane
+// `plane` of item `item` and store the spectrum into plane dest[d][1] of the
+// spectrum of item dest[d][0], d < ndest (identical planes of several block
+// windows are transformed once).
+constexpr int kMaxPlaneDests = 8;
+struct PlaneTask {
+  int item, plane, ndest, pad;
+  int dest[kMaxPlaneDests][2];
+};
 // Fused x-y items: kind kPassXForward (cls = gather) or kPassXInverse (cls = epilogue).
+// tasks (forward only, optional): one PlaneTask per workgroup.
 int build_xy_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls);
 hipError_t launch_batch_xy(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
-                           int nwg, hipStream_t stream);
+                           int nwg, hipStream_t stream, const PlaneTask* tasks = nullptr);
+// Task list of a fused forward launch (see session.hip).
+std::vector<PlaneTask> forward_plane_tasks(const std::vector<const PassDesc*>& items, bool dedup);
 struct ReduceItem {
   const double* part_sum;
   const float* part_max;

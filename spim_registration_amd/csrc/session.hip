@@ -395,8 +395,130 @@ struct PassLaunch {
   bool reduce = false;
   size_t lds = 0;
   DeviceBuf items, prefix, gx;
+  DeviceBuf sched;  // fused forward: one PlaneTask per workgroup
+  bool has_sched = false;
 };
 
+// MVD_XCD_SCHEDULE=0 keeps the item-major order of the forward plane
+// workgroups and one transform per plane; MVD_PLANE_DEDUP=0 keeps the order
+// but transforms every plane of every window.
+static const bool g_xcd_schedule = [] {
+  const char* v = std::getenv("MVD_XCD_SCHEDULE");
+  return !(v && v[0] == '0');
+}();
+static const bool g_plane_dedup = [] {
+  const char* v = std::getenv("MVD_PLANE_DEDUP");
+  return !(v && v[0] == '0');
+}();
+
+static int mirror_single_host(int p, int n) {
+  if (p >= 0 && p < n) return p;
+  if (n == 1) return 0;
+  const int period = 2 * n - 2;
+  p %= period;
+  if (p < 0) p += period;
+  return p < n ? p : period - p;
+}
+
+// Task list of one fused forward launch (one plane of one block window per
+// task).
+//
+// Identical planes.  A window plane's content is fixed by the gather rule,
+// the source, the window's x-y placement and the source plane it reads: under
+// the mirror rule the source plane is the reflection of the window's z; under
+// the constant rules a plane outside the source in z is the constant within
+// the window (whatever its x-y placement), and planes past the window's depth
+// are zero.  Overlapping block windows, and windows that run past an image
+// face, therefore hold many identical planes -- at 512 x 512 x 256 with 256^3
+// blocks and a 51-deep kernel half of each forward launch's 4608 planes.  Each
+// distinct plane is transformed once and stored into every spectrum that holds
+// it (up to kMaxPlaneDests per task, more copies take more tasks); the
+// transform is deterministic, so every copy is bit for bit what its own
+// transform would give.
+//
+// Order.  Workgroup w runs on XCD w mod 8, and each XCD has its own L2.  The
+// planes that read source plane Z (neighbouring windows overlap, reflected
+// windows read planes twice) are sorted by (Z, window y, window x) and the
+// sorted list is cut into 8 contiguous parts, one per XCD, so the workgroups
+// that share a source plane run on one XCD at about the same time.  Planes
+// that read nothing are dealt out evenly at the end of the parts.
+std::vector<PlaneTask> forward_plane_tasks(const std::vector<const PassDesc*>& items, bool dedup) {
+  constexpr int kXcds = 8;
+  enum { kRead = 0, kOutside = 1, kPadding = 2 };
+  // key of a plane's content: (kind, gather, q, win.x, win.y, src, off.x, off.y, source z)
+  using Key = std::tuple<int, int, int, int, int, int, int, uintptr_t, int, int, int>;
+  struct Plane {
+    int z, y, x;  // sort keys: source plane (-1: reads nothing), window y and x
+    std::vector<std::pair<int, int>> dest;
+  };
+  std::vector<Plane> planes;
+  std::map<Key, int> index;
+  for (int i = 0; i < (int)items.size(); ++i) {
+    const PassDesc& d = *items[i];
+    const ConvGeom& g = d.g;
+    for (int p = 0; p < g.q.z; ++p) {
+      int kind = kRead, z = -1;
+      if (p >= g.win.z) {
+        kind = kPadding;
+      } else {
+        const int gz = g.off.z + p;
+        if (d.gather == kGatherMirror) z = mirror_single_host(gz, g.src.z);
+        else if (gz >= 0 && gz < g.src.z) z = gz;
+        else kind = kOutside;
+      }
+      Key k = kind == kRead ? Key{kind, d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y,
+                                  reinterpret_cast<uintptr_t>(d.src), g.off.x, g.off.y, z}
+                            : Key{kind, kind == kPadding ? -1 : d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y,
+                                  0, 0, 0, 0};
+      auto it = dedup ? index.find(k) : index.end();
+      if (it == index.end()) {
+        if (dedup) index.emplace(k, (int)planes.size());
+        planes.push_back({z, g.off.y, g.off.x, {{i, p}}});
+      } else {
+        planes[it->second].dest.push_back({i, p});
+      }
+    }
+  }
+  // tasks: the first destination is the plane that is transformed
+  struct Task {
+    int z, y, x;
+    PlaneTask t;
+  };
+  std::vector<Task> reading, idle;
+  for (auto& pl : planes)
+    for (size_t a = 0; a < pl.dest.size(); a += kMaxPlaneDests) {
+      PlaneTask t{};
+      t.item = pl.dest[a].first;
+      t.plane = pl.dest[a].second;
+      t.ndest = (int)std::min<size_t>(kMaxPlaneDests, pl.dest.size() - a);
+      for (int d = 0; d < kMaxPlaneDests; ++d) {
+        const auto& e = pl.dest[a + std::min(d, t.ndest - 1)];  // unused entries repeat a valid one
+        t.dest[d][0] = e.first;
+        t.dest[d][1] = e.second;
+      }
+      (pl.z >= 0 ? reading : idle).push_back({pl.z, pl.y, pl.x, t});
+    }
+  std::stable_sort(reading.begin(), reading.end(), [](const Task& a, const Task& b) {
+    return a.z != b.z ? a.z < b.z : a.y != b.y ? a.y < b.y : a.x < b.x;
+  });
+  std::vector<std::vector<Task>> part(kXcds);
+  const size_t nr = reading.size();
+  for (int x = 0; x < kXcds; ++x)
+    part[x].assign(reading.begin() + nr * x / kXcds, reading.begin() + nr * (x + 1) / kXcds);
+  for (size_t k = 0; k < idle.size(); ++k) part[k % kXcds].push_back(idle[k]);
+  std::vector<PlaneTask> out;
+  out.reserve(reading.size() + idle.size());
+  for (size_t k = 0;; ++k) {
+    bool any = false;
+    for (int x = 0; x < kXcds; ++x)
+      if (k < part[x].size()) {
+        out.push_back(part[x][k].t);
+        any = true;
+      }
+    if (!any) break;
+  }
+  return out;
+}
 }  // namespace mvd
 
 using namespace mvd;
@@ -538,6 +660,16 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
     MVD_CHECK(hipMemcpy(L.prefix.p, prefix.data(), sizeof(int) * prefix.size(), hipMemcpyHostToDevice));
     MVD_CHECK(L.gx.ensure(sizeof(int) * gxs.size()));
     MVD_CHECK(hipMemcpy(L.gx.p, gxs.data(), sizeof(int) * gxs.size(), hipMemcpyHostToDevice));
+    if (L.path == 2 && L.kind == kPassXForward && g_xcd_schedule) {
+      const std::vector<PlaneTask> tasks = forward_plane_tasks(kv.second, g_plane_dedup);
+      size_t stores = 0;
+      for (const PlaneTask& t : tasks) stores += t.ndest;
+      MVD_REQUIRE((int)stores == L.nwg && !tasks.empty(), "internal: forward task list");
+      MVD_CHECK(L.sched.ensure(sizeof(PlaneTask) * tasks.size()));
+      MVD_CHECK(hipMemcpy(L.sched.p, tasks.data(), sizeof(PlaneTask) * tasks.size(), hipMemcpyHostToDevice));
+      L.nwg = (int)tasks.size();
+      L.has_sched = true;
+    }
     out->push_back(std::move(L));
   }
   return 0;
@@ -547,7 +679,8 @@ hipError_t enqueue_pass_launch(const PassLaunch& L, hipStream_t st) {
   if (L.reduce) return launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, st);
   if (L.lines) return launch_z_lines(L.cls, L.items.p, L.nitems, L.lx, L.ly, st);
   if (L.path == 2)
-    return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
+    return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st,
+                           L.has_sched ? L.sched.as<PlaneTask>() : nullptr);
   if (L.path == 1)
     return launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
   return launch_batch_generic(L.kind, L.epi, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, L.lds,
