@@ -81,6 +81,12 @@ struct ZFastParams {
   const float2* tw;
   float scale, dir;
   int hx, pitch, qy, zlo, zhi;
+  // input planes through a table (lines kernel only): plane z of the item is
+  // read from plane ztab[z] of the arena of spectra, which may hold it for
+  // another item (identical window planes); outputs stay in spec
+  const unsigned short* ztab;
+  const float2* arena;
+  unsigned arena_bytes;
 };
 
 template <int N>
@@ -156,7 +162,12 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 // and stores outside [zlo, zhi) are dropped by the range check rather than
 // branched around, so the wait for the prefetch does not depend on how many
 // stores a lane made.
-template <int N>
+//
+// TABLE: the items' input planes come through their plane tables (the
+// launch's tables, at most kZTabEntries entries, are copied to LDS first);
+// every table entry addresses a plane that no item of the launch overwrites,
+// so the items can run in any order.
+template <int N, bool TABLE>
 __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
@@ -164,9 +175,17 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
+  __shared__ unsigned short ztab[TABLE ? kZTabEntries : 1];
   const ZFastParams P0 = items[0];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P0.tw)[i];
+  if constexpr (TABLE) {
+    for (int it = 0; it < nitems; ++it) {
+      const unsigned short* src = items[it].ztab;
+      for (int i = threadIdx.x; i < N; i += kThreads) ztab[it * N + i] = gmem(src)[i];
+    }
+    __syncthreads();
+  }
   const int col = blockIdx.x * L + line;
   const uint32_t plane = (uint32_t)(P0.qy * P0.pitch) * 8u;  // bytes
   const uint32_t bytes = plane * (uint32_t)N;
@@ -179,18 +198,19 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
   const bool store_col = col < P0.pitch;
   const uint32_t off = (uint32_t)(store_col ? col : P0.hx - 1) * 8u;
   auto addr = [line](int i) { return i * LP + line; };
-  auto load_line = [&](const float2* spec, float2(&v)[E]) {
-    const BufRes r = buf_res(spec, bytes);
+  auto load_line = [&](int it, const float2* spec, float2(&v)[E]) {
+    const BufRes r = TABLE ? buf_res(P0.arena, P0.arena_bytes) : buf_res(spec, bytes);
 #pragma unroll
     for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
       for (int t = 0; t < S::R0; ++t) {
         const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
-        v[b * S::R0 + t] = kStream ? buf_load2_nt(r, z * plane + off, row) : buf_load2(r, z * plane + off, row);
+        const uint32_t zp = TABLE ? (uint32_t)ztab[it * N + z] : z;
+        v[b * S::R0 + t] = kStream ? buf_load2_nt(r, zp * plane + off, row) : buf_load2(r, zp * plane + off, row);
       }
   };
   float2 nxt[E];
-  load_line(P0.spec, nxt);
+  load_line(0, P0.spec, nxt);
   float2 kv[E];
   {
     const BufRes r = buf_res(P0.kspec, bytes);
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = nxt[i];
-    if (it + 1 < nitems) load_line(items[it + 1].spec, nxt);
+    if (it + 1 < nitems) load_line(it + 1, items[it + 1].spec, nxt);
     fft_forward_order<N>(v, lds, jt, twl, 1, -1.f, addr);
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
@@ -233,18 +253,23 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
   }
 }
 
-hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream) {
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, bool table) {
   if (nitems <= 0 || gx <= 0 || gy <= 0) return hipSuccess;
+  if (table && nitems * n > kZTabEntries) return hipErrorInvalidValue;
   (void)hipGetLastError();
   const ZFastParams* it = static_cast<const ZFastParams*>(items);
   const dim3 grid(gx, gy), block(kThreads);
+#define MVD_Z_LINES(NN)                                                                          \
+  if (table) hipLaunchKernelGGL((k_z_fast_lines<NN, true>), grid, block, 0, stream, it, nitems); \
+  else hipLaunchKernelGGL((k_z_fast_lines<NN, false>), grid, block, 0, stream, it, nitems);
   switch (n) {
-    case 64: hipLaunchKernelGGL(k_z_fast_lines<64>, grid, block, 0, stream, it, nitems); break;
-    case 128: hipLaunchKernelGGL(k_z_fast_lines<128>, grid, block, 0, stream, it, nitems); break;
-    case 256: hipLaunchKernelGGL(k_z_fast_lines<256>, grid, block, 0, stream, it, nitems); break;
-    case 512: hipLaunchKernelGGL(k_z_fast_lines<512>, grid, block, 0, stream, it, nitems); break;
+    case 64: MVD_Z_LINES(64) break;
+    case 128: MVD_Z_LINES(128) break;
+    case 256: MVD_Z_LINES(256) break;
+    case 512: MVD_Z_LINES(512) break;
     default: return hipErrorInvalidValue;
   }
+#undef MVD_Z_LINES
   return hipGetLastError();
 }
 
@@ -687,7 +712,8 @@ int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) 
       return sizeof(P);
     }
     case kPassZConv: {
-      ZFastParams P{d.spec, d.kspec, sp.tw_z, d.scale, d.dir, sp.hx, sp.pitch, sp.q.y, d.zlo, d.zhi};
+      ZFastParams P{d.spec, d.kspec, sp.tw_z, d.scale, d.dir, sp.hx, sp.pitch, sp.q.y, d.zlo, d.zhi,
+                    d.ztab, d.arena, d.arena_bytes};
       const int L = fast_lines(sp.q.z);
       *gx = (sp.hx + L - 1) / L;
       *gy = sp.q.y;

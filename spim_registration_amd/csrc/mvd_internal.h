@@ -291,6 +291,11 @@ struct PassDesc {
   int zlo, zhi, ylo, yhi;
   float dir, scale;
   int xy;  // x passes of a fused x-y item (no separate y pass)
+  // z pass in the lines kernel only: input planes through a table of plane
+  // indices into the arena of spectra (nullptr: plane z of spec)
+  const unsigned short* ztab = nullptr;
+  const float2* arena = nullptr;
+  unsigned arena_bytes = 0;
 };
 constexpr int kMaxParamBytes = 640;
 bool pass_is_fast(const PassDesc& d);
@@ -303,8 +308,11 @@ hipError_t launch_batch_generic(int kind, int epi, const void* items, const int*
                                 int nwg, size_t lds, hipStream_t stream);
 hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
                              int nwg, hipStream_t stream);
-// z passes of many blocks sharing one kernel spectrum and line grid (gx, gy).
-hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream);
+// z passes of many blocks sharing one kernel spectrum and line grid (gx, gy);
+// table: the items' input planes come through their plane tables, at most
+// kZTabEntries entries for the launch.
+constexpr int kZTabEntries = 6144;
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, bool table);
 // One workgroup of a fused forward launch with a task list: transform plane
 // `plane` of item `item` and store the spectrum into plane dest[d][1] of the
 // spectrum of item dest[d][0], d < ndest (identical planes of several block

@@ -397,6 +397,7 @@ struct PassLaunch {
   DeviceBuf items, prefix, gx;
   DeviceBuf sched;  // fused forward: one PlaneTask per workgroup
   bool has_sched = false;
+  bool table = false;  // lines z pass reading its input planes through tables
 };
 
 // MVD_XCD_SCHEDULE=0 keeps the item-major order of the forward plane
@@ -442,72 +443,93 @@ static int mirror_single_host(int p, int n) {
 // sorted list is cut into 8 contiguous parts, one per XCD, so the workgroups
 // that share a source plane run on one XCD at about the same time.  Planes
 // that read nothing are dealt out evenly at the end of the parts.
-std::vector<PlaneTask> forward_plane_tasks(const std::vector<const PassDesc*>& items, bool dedup) {
-  constexpr int kXcds = 8;
-  enum { kRead = 0, kOutside = 1, kPadding = 2 };
-  // key of a plane's content: (kind, gather, q, win.x, win.y, src, off.x, off.y, source z)
-  using Key = std::tuple<int, int, int, int, int, int, int, uintptr_t, int, int, int>;
-  struct Plane {
-    int z, y, x;  // sort keys: source plane (-1: reads nothing), window y and x
-    std::vector<std::pair<int, int>> dest;
-  };
-  std::vector<Plane> planes;
-  std::map<Key, int> index;
+namespace {
+
+enum { kPlaneRead = 0, kPlaneOutside = 1, kPlanePadding = 2 };
+// key of a window plane's content: (kind, gather, q, win.x, win.y, src, off.x, off.y, source z)
+using PlaneKey = std::tuple<int, int, int, int, int, int, int, uintptr_t, int, int, int>;
+
+// Content key of plane p of a forward item; *z = source plane read (-1: none).
+PlaneKey plane_key(const PassDesc& d, int p, int* z) {
+  const ConvGeom& g = d.g;
+  int kind = kPlaneRead;
+  *z = -1;
+  if (p >= g.win.z) {
+    kind = kPlanePadding;
+  } else {
+    const int gz = g.off.z + p;
+    if (d.gather == kGatherMirror) *z = mirror_single_host(gz, g.src.z);
+    else if (gz >= 0 && gz < g.src.z) *z = gz;
+    else kind = kPlaneOutside;
+  }
+  if (kind == kPlaneRead)
+    return PlaneKey{kind, d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y, reinterpret_cast<uintptr_t>(d.src),
+                    g.off.x, g.off.y, *z};
+  return PlaneKey{kind, kind == kPlanePadding ? -1 : d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y, 0, 0, 0, 0};
+}
+
+// Distinct planes of a forward launch: per content key, the (item, plane)
+// occurrences in item and plane order, and the source plane and window
+// placement used to order the tasks.
+struct PlaneGroup {
+  int z, y, x;
+  std::vector<std::pair<int, int>> occ;
+};
+std::vector<PlaneGroup> plane_groups(const std::vector<const PassDesc*>& items, bool dedup) {
+  std::vector<PlaneGroup> groups;
+  std::map<PlaneKey, int> index;
   for (int i = 0; i < (int)items.size(); ++i) {
     const PassDesc& d = *items[i];
-    const ConvGeom& g = d.g;
-    for (int p = 0; p < g.q.z; ++p) {
-      int kind = kRead, z = -1;
-      if (p >= g.win.z) {
-        kind = kPadding;
-      } else {
-        const int gz = g.off.z + p;
-        if (d.gather == kGatherMirror) z = mirror_single_host(gz, g.src.z);
-        else if (gz >= 0 && gz < g.src.z) z = gz;
-        else kind = kOutside;
-      }
-      Key k = kind == kRead ? Key{kind, d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y,
-                                  reinterpret_cast<uintptr_t>(d.src), g.off.x, g.off.y, z}
-                            : Key{kind, kind == kPadding ? -1 : d.gather, g.q.x, g.q.y, g.q.z, g.win.x, g.win.y,
-                                  0, 0, 0, 0};
+    for (int p = 0; p < d.g.q.z; ++p) {
+      int z;
+      const PlaneKey k = plane_key(d, p, &z);
       auto it = dedup ? index.find(k) : index.end();
       if (it == index.end()) {
-        if (dedup) index.emplace(k, (int)planes.size());
-        planes.push_back({z, g.off.y, g.off.x, {{i, p}}});
+        if (dedup) index.emplace(k, (int)groups.size());
+        groups.push_back({z, d.g.off.y, d.g.off.x, {{i, p}}});
       } else {
-        planes[it->second].dest.push_back({i, p});
+        groups[it->second].occ.push_back({i, p});
       }
     }
   }
-  // tasks: the first destination is the plane that is transformed
-  struct Task {
-    int z, y, x;
-    PlaneTask t;
-  };
-  std::vector<Task> reading, idle;
-  for (auto& pl : planes)
-    for (size_t a = 0; a < pl.dest.size(); a += kMaxPlaneDests) {
-      PlaneTask t{};
-      t.item = pl.dest[a].first;
-      t.plane = pl.dest[a].second;
-      t.ndest = (int)std::min<size_t>(kMaxPlaneDests, pl.dest.size() - a);
-      for (int d = 0; d < kMaxPlaneDests; ++d) {
-        const auto& e = pl.dest[a + std::min(d, t.ndest - 1)];  // unused entries repeat a valid one
-        t.dest[d][0] = e.first;
-        t.dest[d][1] = e.second;
-      }
-      (pl.z >= 0 ? reading : idle).push_back({pl.z, pl.y, pl.x, t});
+  return groups;
+}
+
+struct OrderedTask {
+  int z, y, x;
+  PlaneTask t;
+};
+// Tasks storing the plane of group g into occurrences [a, a + n) of dest.
+void push_tasks(const PlaneGroup& g, const std::vector<std::pair<int, int>>& dest, std::vector<OrderedTask>* out) {
+  for (size_t a = 0; a < dest.size(); a += kMaxPlaneDests) {
+    PlaneTask t{};
+    t.item = dest[a].first;
+    t.plane = dest[a].second;
+    t.ndest = (int)std::min<size_t>(kMaxPlaneDests, dest.size() - a);
+    for (int d = 0; d < kMaxPlaneDests; ++d) {
+      const auto& e = dest[a + std::min(d, t.ndest - 1)];  // unused entries repeat a valid one
+      t.dest[d][0] = e.first;
+      t.dest[d][1] = e.second;
     }
-  std::stable_sort(reading.begin(), reading.end(), [](const Task& a, const Task& b) {
+    out->push_back({g.z, g.y, g.x, t});
+  }
+}
+
+// XCD-aware order (see forward_plane_tasks).
+std::vector<PlaneTask> order_tasks(std::vector<OrderedTask> tasks) {
+  constexpr int kXcds = 8;
+  std::vector<OrderedTask> reading, idle;
+  for (auto& t : tasks) (t.z >= 0 ? reading : idle).push_back(t);
+  std::stable_sort(reading.begin(), reading.end(), [](const OrderedTask& a, const OrderedTask& b) {
     return a.z != b.z ? a.z < b.z : a.y != b.y ? a.y < b.y : a.x < b.x;
   });
-  std::vector<std::vector<Task>> part(kXcds);
+  std::vector<std::vector<OrderedTask>> part(kXcds);
   const size_t nr = reading.size();
   for (int x = 0; x < kXcds; ++x)
     part[x].assign(reading.begin() + nr * x / kXcds, reading.begin() + nr * (x + 1) / kXcds);
   for (size_t k = 0; k < idle.size(); ++k) part[k % kXcds].push_back(idle[k]);
   std::vector<PlaneTask> out;
-  out.reserve(reading.size() + idle.size());
+  out.reserve(tasks.size());
   for (size_t k = 0;; ++k) {
     bool any = false;
     for (int x = 0; x < kXcds; ++x)
@@ -518,6 +540,59 @@ std::vector<PlaneTask> forward_plane_tasks(const std::vector<const PassDesc*>& i
     if (!any) break;
   }
   return out;
+}
+
+}  // namespace
+
+std::vector<PlaneTask> forward_plane_tasks(const std::vector<const PassDesc*>& items, bool dedup) {
+  std::vector<OrderedTask> tasks;
+  for (const PlaneGroup& g : plane_groups(items, dedup)) push_tasks(g, g.occ, &tasks);
+  return order_tasks(std::move(tasks));
+}
+
+// Forward tasks and z-pass plane tables of one convolution of a view whose
+// block spectra lie in one arena (block-batched iteration).  A distinct plane
+// is transformed once when one of its occurrences lies outside its own
+// block's output planes [zlo, zhi): that copy is never overwritten by a z
+// pass, so every other occurrence reads it through its table and the z passes
+// can run in any order.  A plane whose occurrences are all output planes is
+// stored into each of them (as forward_plane_tasks does).  ztab: per z item
+// (in the order of zp) q.z plane indices into the arena.
+bool plan_conv_planes(const std::vector<PassDesc>& fwd, const std::vector<PassDesc>& zp, const float2* arena,
+                      int64_t elems, std::vector<PlaneTask>* tasks, std::vector<unsigned short>* ztab) {
+  if (fwd.empty() || fwd.size() != zp.size() || !arena || elems <= 0) return false;
+  const int qz = fwd[0].g.q.z;
+  std::vector<const PassDesc*> items;
+  std::vector<int> slot;
+  for (size_t i = 0; i < fwd.size(); ++i) {
+    if (fwd[i].g.q.z != qz || fwd[i].spec != zp[i].spec) return false;
+    const int64_t off = fwd[i].spec - arena;
+    if (off < 0 || off % elems) return false;
+    slot.push_back((int)(off / elems));
+    items.push_back(&fwd[i]);
+  }
+  int max_slot = 0;
+  for (int sl : slot) max_slot = std::max(max_slot, sl);
+  if ((int64_t)(max_slot + 1) * qz > 65535 || (int64_t)fwd.size() * qz > kZTabEntries) return false;
+  ztab->assign(fwd.size() * qz, 0);
+  std::vector<OrderedTask> out;
+  for (const PlaneGroup& g : plane_groups(items, g_plane_dedup)) {
+    const std::pair<int, int>* keep = nullptr;
+    for (const auto& o : g.occ)
+      if (o.second < zp[o.first].zlo || o.second >= zp[o.first].zhi) {
+        keep = &o;
+        break;
+      }
+    if (keep) {
+      push_tasks(g, {*keep}, &out);
+      for (const auto& o : g.occ) (*ztab)[o.first * qz + o.second] = (unsigned short)(slot[keep->first] * qz + keep->second);
+    } else {
+      push_tasks(g, g.occ, &out);
+      for (const auto& o : g.occ) (*ztab)[o.first * qz + o.second] = (unsigned short)(slot[o.first] * qz + o.second);
+    }
+  }
+  *tasks = order_tasks(std::move(out));
+  return true;
 }
 }  // namespace mvd
 
@@ -556,7 +631,8 @@ struct mvd_session {
   // spectra do not fit (the per-block loop then reuses d_spec).
   struct BlockBatch {
     bool built = false, enabled = true;
-    std::vector<DeviceBuf> spec;
+    DeviceBuf arena;                 // the block spectra, one after another
+    std::vector<DeviceBuf> ztab;     // z-pass plane tables, per view and convolution
     std::vector<std::vector<PassLaunch>> view;  // per view: convolve1 passes, then convolve2 passes
   } bb;
 
@@ -592,7 +668,10 @@ int timing_class(int kind, int path, int epi) {
 
 // Groups the items of one pass by kernel class -- (path, template selector,
 // epilogue) -- and appends one launch per group.
-int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaunch>* out) {
+// fwd_tasks: the task list of the fused forward group when the caller planned
+// it (plan_conv_planes); otherwise forward_plane_tasks makes one.
+int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaunch>* out,
+                        const std::vector<PlaneTask>* fwd_tasks = nullptr) {
   // z items (register kernels) that share a kernel spectrum and shape with at
   // least one other item form their own group: one "lines" launch reads the
   // kernel spectrum once for all of them (the blocks of one view of one stack)
@@ -648,6 +727,10 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       int cls = 0;
       build_fast_params(*kv.second.front(), tmp, &L.lx, &L.ly, &cls);
       L.lines = true;
+      int with_table = 0;
+      for (const PassDesc* d : kv.second) with_table += d->ztab != nullptr;
+      MVD_REQUIRE(with_table == 0 || with_table == L.nitems, "internal: z items with and without plane tables");
+      L.table = with_table > 0;
       // the lines kernel loads an item's lines while it stores the previous
       // item's: the spectra must be distinct buffers
       std::set<const float2*> specs;
@@ -661,10 +744,13 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
     MVD_CHECK(L.gx.ensure(sizeof(int) * gxs.size()));
     MVD_CHECK(hipMemcpy(L.gx.p, gxs.data(), sizeof(int) * gxs.size(), hipMemcpyHostToDevice));
     if (L.path == 2 && L.kind == kPassXForward && g_xcd_schedule) {
-      const std::vector<PlaneTask> tasks = forward_plane_tasks(kv.second, g_plane_dedup);
+      const std::vector<PlaneTask> tasks = fwd_tasks ? *fwd_tasks : forward_plane_tasks(kv.second, g_plane_dedup);
       size_t stores = 0;
       for (const PlaneTask& t : tasks) stores += t.ndest;
-      MVD_REQUIRE((int)stores == L.nwg && !tasks.empty(), "internal: forward task list");
+      // every plane is stored once, unless the caller planned plane tables
+      // (then shared planes are stored only where the tables read them)
+      MVD_REQUIRE(!tasks.empty() && (fwd_tasks ? (int)stores <= L.nwg : (int)stores == L.nwg),
+                  "internal: forward task list");
       MVD_CHECK(L.sched.ensure(sizeof(PlaneTask) * tasks.size()));
       MVD_CHECK(hipMemcpy(L.sched.p, tasks.data(), sizeof(PlaneTask) * tasks.size(), hipMemcpyHostToDevice));
       L.nwg = (int)tasks.size();
@@ -677,7 +763,7 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
 
 hipError_t enqueue_pass_launch(const PassLaunch& L, hipStream_t st) {
   if (L.reduce) return launch_reduce_stats_batch(L.items.as<ReduceItem>(), L.nitems, st);
-  if (L.lines) return launch_z_lines(L.cls, L.items.p, L.nitems, L.lx, L.ly, st);
+  if (L.lines) return launch_z_lines(L.cls, L.items.p, L.nitems, L.lx, L.ly, st, L.table);
   if (L.path == 2)
     return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st,
                            L.has_sched ? L.sched.as<PlaneTask>() : nullptr);
@@ -764,32 +850,48 @@ void conv_pass_descs(mvd_session* s, int v, int conv, const std::vector<float2*>
 int build_block_batch(mvd_session* s) {
   auto& bb = s->bb;
   bb.view.clear();
+  bb.ztab.clear();
   size_t nb = 0, elems = 0;
   for (auto& v : s->views) {
     nb = std::max(nb, v.blocks.size());
     elems = std::max<size_t>(elems, v.plan->spectrum_elems());
   }
-  size_t have = 0, free_b = 0, total_b = 0;
-  for (auto& b : bb.spec) have += b.bytes;
+  size_t free_b = 0, total_b = 0;
   MVD_CHECK(hipMemGetInfo(&free_b, &total_b));
   const size_t need = nb * elems * sizeof(float2);
   bb.built = true;
-  bb.enabled = nb > 1 && need <= have + free_b / 2;
+  bb.enabled = nb > 1 && need <= bb.arena.bytes + free_b / 2;
   if (!bb.enabled) return 0;
-  bb.spec.resize(nb);
+  MVD_CHECK(bb.arena.ensure(need));
+  MVD_CHECK(hipMemsetAsync(bb.arena.p, 0, need, s->stream));  // finite row padding
   std::vector<float2*> spec;
-  for (auto& b : bb.spec) {
-    MVD_CHECK(b.ensure(sizeof(float2) * elems));
-    MVD_CHECK(hipMemsetAsync(b.p, 0, sizeof(float2) * elems, s->stream));  // finite row padding
-    spec.push_back(b.as<float2>());
-  }
+  for (size_t b = 0; b < nb; ++b) spec.push_back(bb.arena.as<float2>() + b * elems);
+  // the lines kernel addresses the arena with 32-bit offsets
+  const bool tables = g_xcd_schedule && need < (size_t(1) << 31);
   bb.view.resize(s->nviews);
   for (int v = 0; v < s->nviews; ++v)
     for (int conv = 0; conv < 2; ++conv) {
       std::vector<PassDesc> pass[5];
       conv_pass_descs(s, v, conv, spec, pass);
+      std::vector<PlaneTask> tasks;
+      std::vector<unsigned short> tab;
+      const bool planned = tables && pass[0].size() == pass[2].size() && !pass[0].empty() && pass[0][0].xy &&
+                           pass_is_fast(pass[2][0]) &&
+                           plan_conv_planes(pass[0], pass[2], bb.arena.as<float2>(), (int64_t)elems, &tasks, &tab);
+      if (planned) {
+        bb.ztab.emplace_back();
+        DeviceBuf& dt = bb.ztab.back();
+        MVD_CHECK(dt.ensure(sizeof(unsigned short) * tab.size()));
+        MVD_CHECK(hipMemcpy(dt.p, tab.data(), sizeof(unsigned short) * tab.size(), hipMemcpyHostToDevice));
+        const int qz = pass[2][0].g.q.z;
+        for (size_t i = 0; i < pass[2].size(); ++i) {
+          pass[2][i].ztab = dt.as<unsigned short>() + i * qz;
+          pass[2][i].arena = bb.arena.as<float2>();
+          pass[2][i].arena_bytes = (unsigned)need;
+        }
+      }
       for (int p = 0; p < 5; ++p) {
-        int rc = build_pass_launches(pass[p], &bb.view[v]);
+        int rc = build_pass_launches(pass[p], &bb.view[v], p == 0 && planned ? &tasks : nullptr);
         if (rc) return rc;
       }
     }
