@@ -243,6 +243,7 @@ class MVDeconvolution:
         v0 = vs[0]
         self.name = name
         self.views = views
+        self.norm_threads = int(norm_threads)
         self.num_iterations = int(num_iterations)
         self.lam = float(lam)
         self.i = 0

@@ -17,6 +17,18 @@ with S_v(j) taken as the total weighted estimate (exact away from the image
 border, where the mirror extension repeats the estimate), eta = 1 the plain
 EM step, clipped at zero and renormalised to unit sum like every kernel the
 reference accepts (normImg in MVDeconFFT.init).
+
+The likelihood alone does not fix the PSFs: the estimate blurred by a common
+kernel G with every PSF sharpened by G fits the data about as well, so
+descent on the free PSF values (or on a PSF model's width) drifts towards
+sharper PSFs and a blurrier estimate.  What the data of several views do fix
+is how the views' PSFs differ from one another, since every view sees the
+same estimate.  `refine_psfs_parametric` uses that: the PSFs come from a model
+with positive parameters per view (e.g. the widths of a rotated Gaussian), the
+gradient in the log-parameters is the chain rule through the model of the GPU
+PSF gradient, its mean over the views -- the common scale the likelihood
+cannot identify -- is removed, and each step is backtracked until the NLL at
+the current estimate does not rise.
 """
 from __future__ import annotations
 
@@ -26,6 +38,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from .deconvolution import MVDeconvolution
+from . import synthetic
 
 
 @dataclasses.dataclass
@@ -81,4 +94,87 @@ def refine_psfs(decon: MVDeconvolution, steps: int, iterations: int = 1, eta: fl
         # kernels couple the views)
         decon.set_kernels(new_psfs)
         history.append(RefinementStep(step, nll, change))
+    return history
+
+
+@dataclasses.dataclass
+class ParametricStep:
+    step: int
+    nll: float              # NLL after the step's deconvolution iterations, at the PSFs it started from
+    params: np.ndarray      # per view, after the step
+    accepted: bool          # the backtracking found a step that does not raise the NLL
+
+
+def gaussian_psf_model(k_xyz, angles: Sequence[float]):
+    """Model of the synthetic PSFs: view v's PSF is a normalised anisotropic
+    Gaussian (parameters sigma_xy, sigma_z) rotated by angles[v] about y, in
+    the box k_xyz."""
+    def model(view: int, params) -> np.ndarray:
+        return synthetic.psf(k_xyz, float(params[0]), float(params[1]), angles[view])
+    return model
+
+
+def _norm_img(p: np.ndarray, threads: int) -> np.ndarray:
+    """The kernel normalisation of the deconvolution (AdjustInput.normImg,
+    deconv/AdjustInput.java:50-121): divided by its sum plus the sum of its
+    first portion of 2 * threads (float64 sums; the reference's pairwise
+    summation order moves only the last bits)."""
+    q = p.astype(np.float64).ravel()
+    first = q[: q.size // (2 * threads)].sum()
+    return (q / (q.sum() + first)).astype(np.float32).reshape(p.shape)
+
+
+def refine_psfs_parametric(decon: MVDeconvolution, params, model, steps: int, iterations: int = 1,
+                           rate: float = 0.05, rel_h: float = 1e-3, max_halvings: int = 12) -> List[ParametricStep]:
+    """Interleave `iterations` deconvolution iterations and one update of the
+    PSF model parameters, `steps` times.  params: [views][n] positive
+    parameters; model(view, params) -> PSF (the kernel box, float32).
+
+    Per step: dL/dP_v from the GPU, dL/dlog(theta_v,j) = theta_v,j sum_k
+    dL/dP_v(k) dP_v(k)/dtheta_v,j (central differences of the unit-sum model,
+    step rel_h, through normImg), the mean over the views removed per
+    parameter, and log theta <- log theta - t rate g / max_v |g| with t halved
+    until the NLL at the current estimate does not rise.  The views' PSFs are set through set_kernels (normImg, kernel
+    preparation of all views)."""
+    views = decon.get_data().get_views()
+    nt = decon.norm_threads
+    theta = np.array(params, np.float64)
+    if theta.ndim != 2 or theta.shape[0] != len(views) or np.any(theta <= 0):
+        raise ValueError("params: one row of positive parameters per view")
+    decon.set_kernels([model(v, theta[v]) for v in range(len(views))])
+    history: List[ParametricStep] = []
+    for step in range(steps):
+        decon.iterate(iterations)
+        # the kernels the iteration uses (kernel1 after normImg)
+        cur = [view.get_kernel1() for view in views]
+        nll, _, g = decon.nll_grad(psfs=cur, want_psi=False, want_psf=True)
+        grad = np.zeros_like(theta)
+        for v in range(len(views)):
+            gv = g[v].astype(np.float64)
+            for j in range(theta.shape[1]):
+                h = rel_h * theta[v, j]
+                up, dn = theta[v].copy(), theta[v].copy()
+                up[j] += h
+                dn[j] -= h
+                dp = (_norm_img(model(v, up), nt).astype(np.float64) -
+                      _norm_img(model(v, dn), nt).astype(np.float64)) / (2 * h)
+                grad[v, j] = theta[v, j] * float((gv * dp).sum())
+        grad -= grad.mean(axis=0, keepdims=True)
+        scale = np.abs(grad).max(axis=0)
+        scale[scale == 0] = 1.0
+        direction = -grad / scale
+        accepted = False
+        t = 1.0
+        for _ in range(max_halvings):
+            cand = theta * np.exp(t * rate * direction)
+            new = [_norm_img(model(v, cand[v]), nt) for v in range(len(views))]
+            n2 = decon.nll_grad(psfs=new, want_psi=False, want_psf=False)[0]
+            if n2 <= nll:
+                theta = cand
+                accepted = True
+                break
+            t *= 0.5
+        if accepted:
+            decon.set_kernels([model(v, theta[v]) for v in range(len(views))])
+        history.append(ParametricStep(step, nll, theta.copy(), accepted))
     return history

@@ -81,3 +81,11 @@ def test_interleaved_steps_lower_nll_and_approach_true_psfs():
         prev = _nll(ds, psfs, psi)[0]
         assert prev <= nll + 1e-9 * abs(nll)
     assert err(psfs) < 0.9 * e0
+
+
+def test_norm_img_restatement_matches_oracle():
+    from spim_registration_amd.refinement import _norm_img
+    p = S.psf((9, 11, 13), 1.1, 3.3, 30)
+    for threads in (1, 4, 8):
+        ref = O.norm_img(p, threads)
+        np.testing.assert_allclose(_norm_img(p, threads), ref, rtol=2e-7, atol=0)
