@@ -124,6 +124,26 @@ def _norm_img(p: np.ndarray, threads: int) -> np.ndarray:
     return (q / (q.sum() + first)).astype(np.float32).reshape(p.shape)
 
 
+def log_param_gradient(gpsf: Sequence[np.ndarray], theta, model, threads: int, rel_h: float = 1e-3) -> np.ndarray:
+    """dL/dlog(theta_v,j) = theta_v,j sum_k dL/dP_v(k) dP_v(k)/dtheta_v,j for
+    PSFs P_v = normImg(model(v, theta_v)), with dP/dtheta by central
+    differences of the normalised model (step rel_h * theta_v,j).  gpsf: the
+    PSF gradient per view at those PSFs (float64 arithmetic)."""
+    theta = np.asarray(theta, np.float64)
+    grad = np.zeros_like(theta)
+    for v in range(theta.shape[0]):
+        gv = np.asarray(gpsf[v], np.float64)
+        for j in range(theta.shape[1]):
+            h = rel_h * theta[v, j]
+            up, dn = theta[v].copy(), theta[v].copy()
+            up[j] += h
+            dn[j] -= h
+            dp = (_norm_img(model(v, up), threads).astype(np.float64) -
+                  _norm_img(model(v, dn), threads).astype(np.float64)) / (2 * h)
+            grad[v, j] = theta[v, j] * float((gv * dp).sum())
+    return grad
+
+
 def refine_psfs_parametric(decon: MVDeconvolution, params, model, steps: int, iterations: int = 1,
                            rate: float = 0.05, rel_h: float = 1e-3, max_halvings: int = 12) -> List[ParametricStep]:
     """Interleave `iterations` deconvolution iterations and one update of the
@@ -148,17 +168,7 @@ def refine_psfs_parametric(decon: MVDeconvolution, params, model, steps: int, it
         # the kernels the iteration uses (kernel1 after normImg)
         cur = [view.get_kernel1() for view in views]
         nll, _, g = decon.nll_grad(psfs=cur, want_psi=False, want_psf=True)
-        grad = np.zeros_like(theta)
-        for v in range(len(views)):
-            gv = g[v].astype(np.float64)
-            for j in range(theta.shape[1]):
-                h = rel_h * theta[v, j]
-                up, dn = theta[v].copy(), theta[v].copy()
-                up[j] += h
-                dn[j] -= h
-                dp = (_norm_img(model(v, up), nt).astype(np.float64) -
-                      _norm_img(model(v, dn), nt).astype(np.float64)) / (2 * h)
-                grad[v, j] = theta[v, j] * float((gv * dp).sum())
+        grad = log_param_gradient(g, theta, model, nt, rel_h)
         grad -= grad.mean(axis=0, keepdims=True)
         scale = np.abs(grad).max(axis=0)
         scale[scale == 0] = 1.0

@@ -89,3 +89,36 @@ def test_norm_img_restatement_matches_oracle():
     for threads in (1, 4, 8):
         ref = O.norm_img(p, threads)
         np.testing.assert_allclose(_norm_img(p, threads), ref, rtol=2e-7, atol=0)
+
+
+def test_log_param_gradient_matches_nll_differences():
+    """The parametric rule's gradient (the oracle's PSF gradient through the
+    unit-sum Gaussian model) equals central differences of the oracle's NLL in
+    the log-parameters; a gradient that skips the normalisation's derivative
+    (a plausible slip) does not."""
+    from spim_registration_amd.refinement import _norm_img, gaussian_psf_model, log_param_gradient
+    ds, _, psi = _case(seed=5)
+    k = ds.psfs[0].shape[::-1]
+    model = gaussian_psf_model(k, ds.angles)
+    theta = np.array([[1.1, 3.3], [1.35, 2.6]])
+    threads = 4
+
+    def psfs_at(t):
+        return [_norm_img(model(v, t[v]), threads).astype(np.float64) for v in range(len(t))]
+
+    _, _, g = _nll(ds, psfs_at(theta), psi)
+    got = log_param_gradient(g, theta, model, threads, rel_h=1e-4)
+    h = 1e-4
+    for v in range(2):
+        for j in range(2):
+            up, dn = theta.copy(), theta.copy()
+            up[v, j] *= np.exp(h)
+            dn[v, j] *= np.exp(-h)
+            fd = (_nll(ds, psfs_at(up), psi)[0] - _nll(ds, psfs_at(dn), psi)[0]) / (2 * h)
+            assert abs(got[v, j] - fd) < 2e-3 * np.abs(got).max(), (v, j, got[v, j], fd)
+    # the raw model without normImg's derivative misses the flux term
+    raw = np.array([[theta[v, j] * float((g[v] * (model(v, theta[v] * np.exp(1e-4 * (np.arange(2) == j))) -
+                                                  model(v, theta[v] * np.exp(-1e-4 * (np.arange(2) == j)))
+                                                  ).astype(np.float64)).sum()) / (2e-4 * theta[v, j])
+                     for j in range(2)] for v in range(2)])
+    assert np.abs(raw - got).max() > 1e-2 * np.abs(got).max()
