@@ -283,8 +283,9 @@ def run_c4(args):
     torch.cuda.synchronize()
     rsec = time.perf_counter() - t0
     e1 = psf_err()
-    log(f"{args.refine_steps} refinement steps in {rsec:.2f} s; PSF rel. error {e0:.4f} -> {e1:.4f}; "
-        f"NLL {hist[0].nll:.6e} -> {hist[-1].nll:.6e}")
+    if hist:
+        log(f"{args.refine_steps} refinement steps in {rsec:.2f} s; PSF rel. error {e0:.4f} -> {e1:.4f}; "
+            f"NLL {hist[0].nll:.6e} -> {hist[-1].nll:.6e}")
     # the parametric rule (per-view Gaussian widths, common scale kept) on a
     # fresh deconvolution from the same start
     from spim_registration_amd.refinement import gaussian_psf_model, refine_psfs_parametric
@@ -311,8 +312,9 @@ def run_c4(args):
     pe1 = psf_err2()
     pnll = [h.nll for h in phist]
     pmono = all(b <= a + 1e-9 * abs(a) for a, b in zip(pnll, pnll[1:]))
+    theta1 = phist[-1].params if phist else theta0
     log(f"parametric refinement: {args.refine_steps} steps in {psec:.2f} s; PSF rel. error {pe0:.4f} -> {pe1:.4f}; "
-        f"NLL monotone {pmono}; widths {np.round(phist[-1].params, 3).tolist()}")
+        f"NLL monotone {pmono}; widths {np.round(theta1, 3).tolist()}")
     d2.close()
     cpu = None
     if not args.no_cpu_baseline:
@@ -368,7 +370,7 @@ def run_c4(args):
         "refinement_parametric": {
             "steps": args.refine_steps, "seconds": round(psec, 3), "psf_rel_err_start": round(pe0, 5),
             "psf_rel_err_end": round(pe1, 5), "nll_monotone": bool(pmono),
-            "widths_start": np.round(theta0, 4).tolist(), "widths_end": np.round(phist[-1].params, 4).tolist(),
+            "widths_start": np.round(theta0, 4).tolist(), "widths_end": np.round(theta1, 4).tolist(),
             "widths_true": [w["sxy"], w["sz"]],
             "rule": "per-view Gaussian widths; gradient in log-widths through the model, its mean over the views "
                     "(the common blur the likelihood cannot identify) removed; backtracked on the NLL"},

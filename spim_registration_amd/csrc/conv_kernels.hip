@@ -152,6 +152,7 @@ __global__ __launch_bounds__(kThreads) void k_y_batch(const YParams* items, cons
 struct ZParams {
   float2* spec;
   const float2* kspec;
+  float2* aux;  // with kspec: the forward spectrum before the product, all rows (nullable)
   float scale, dir;
   int hx, pitch, qy, qz, zlo, zhi, lines;
   FftPlan plan;
@@ -178,7 +179,11 @@ __device__ __forceinline__ void z_body(const ZParams& P, int bx, int by) {
   if (P.kspec) {
     for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
       const int z = idx / B, b = idx - z * B;
-      if (c0 + b < P.hx) buf[z * lp + b] = cmul(buf[z * lp + b], gmem(P.kspec)[z * plane + col + b]);
+      if (c0 + b < P.hx) {
+        const float2 v = buf[z * lp + b];
+        if (P.aux) gmem(P.aux)[z * plane + col + b] = v;
+        buf[z * lp + b] = cmul(v, gmem(P.kspec)[z * plane + col + b]);
+      }
     }
     __syncthreads();
     lds_fft(buf, lp, B, P.plan, tw, 1, 1.f);
@@ -196,6 +201,89 @@ __device__ __forceinline__ void z_body(const ZParams& P, int bx, int by) {
 }
 
 __global__ __launch_bounds__(kThreads) void k_z(ZParams P) { z_body(P, blockIdx.x, blockIdx.y); }
+
+// z pass of the likelihood gradient, after the x and y transforms of a block's
+// residual R.  Per line: the forward transform R^ (z), then
+//   adjoint     A = R^ conj(P^)  -> inverse z -> rows [zlo, zhi) of spec (in
+//               place; skipped when zhi <= zlo);
+//   correlation C = conj(R^) W^ e(-2 pi i <k, s> / q), the lag shift s moving
+//               the lags the kernel needs, c - j for j in the kernel box, to
+//               [0, kd): accumulated into corr over the blocks (mode 0), or
+//               inverse-transformed along z with rows [0, kz) stored (mode 1);
+//               skipped when corr is null.
+// LDS holds the two line sets one after the other.
+struct ZGradParams {
+  float2* spec;
+  const float2* what;
+  const float2* khat;
+  float2* corr;
+  int mode, first;
+  Int3 shift, q;
+  int hx, pitch, zlo, zhi, kz, lines;
+  FftPlan plan;
+  const float2* tw;
+  const float2* twx;  // exp(-2 pi i m / q.x), m < q.x
+  const float2* twy;  // exp(-2 pi i m / q.y), m < q.y
+};
+
+__global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
+  extern __shared__ float2 smem[];
+  const int B = P.lines, lp = B + 1, qz = P.q.z;
+  float2* tw = smem;
+  float2* buf = smem + qz;
+  float2* cbuf = buf + qz * lp;
+  for (int i = threadIdx.x; i < qz; i += kThreads) tw[i] = gmem(P.tw)[i];
+  const int c0 = blockIdx.x * B;
+  const int y = blockIdx.y;
+  const int64_t plane = (int64_t)P.q.y * P.pitch;
+  const int64_t col = (int64_t)y * P.pitch + c0;
+  for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
+    const int z = idx / B, b = idx - z * B;
+    buf[z * lp + b] = make_float2(0.f, 0.f);
+    if (c0 + b < P.hx) buf[z * lp + b] = gmem(P.spec)[z * plane + col + b];
+  }
+  __syncthreads();
+  lds_fft(buf, lp, B, P.plan, tw, 1, -1.f);
+  const bool adj = P.zhi > P.zlo, corr = P.corr != nullptr;
+  // per-line part of the lag-shift phase (kx = c0 + b, ky = y)
+  const int ty = (int)(((int64_t)y * P.shift.y) % P.q.y);
+  const float2 py = gmem(P.twy)[ty];
+  for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
+    const int z = idx / B, b = idx - z * B;
+    const int kx = c0 + b;
+    float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
+    if (kx < P.hx) {
+      const int64_t o = z * plane + col + b;
+      const float2 r = buf[z * lp + b];
+      if (adj) a = cmul(r, cconj(gmem(P.khat)[o]));
+      if (corr) {
+        const int tx = (int)(((int64_t)kx * P.shift.x) % P.q.x);
+        const int tz = (int)(((int64_t)z * P.shift.z) % qz);
+        const float2 ph = cmul(cmul(gmem(P.twx)[tx], py), tw[tz]);
+        c = cmul(cmul(cconj(r), gmem(P.what)[o]), ph);
+        if (P.mode == 0) {
+          if (!P.first) c = cadd(c, gmem(P.corr)[o]);
+          gmem(P.corr)[o] = c;
+        }
+      }
+    }
+    buf[z * lp + b] = a;
+    cbuf[z * lp + b] = c;
+  }
+  __syncthreads();
+  if (adj) lds_fft(buf, lp, B, P.plan, tw, 1, 1.f);
+  if (corr && P.mode == 1) lds_fft(cbuf, lp, B, P.plan, tw, 1, 1.f);
+  for (int idx = threadIdx.x; idx < (P.zhi - P.zlo) * B; idx += kThreads) {
+    const int z = P.zlo + idx / B, b = idx % B;
+    if (c0 + b < P.hx) gmem(P.spec)[z * plane + col + b] = buf[z * lp + b];
+  }
+  if (corr && P.mode == 1) {
+    for (int idx = threadIdx.x; idx < P.kz * B; idx += kThreads) {
+      const int z = idx / B, b = idx % B;
+      if (c0 + b < P.hx) gmem(P.corr)[z * plane + col + b] = cbuf[z * lp + b];
+    }
+  }
+}
 
 __global__ __launch_bounds__(kThreads) void k_z_batch(const ZParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
@@ -378,9 +466,13 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
 }
 
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream, float dir) {
-  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir);
+                    hipStream_t stream, float dir, float2* aux) {
+  if (fast_size(sp.q.z)) {
+    if (aux) return hipErrorInvalidValue;
+    return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir);
+  }
   ZParams P;
+  P.aux = aux;
   P.dir = dir;
   P.spec = spec;
   P.kspec = kspec;
@@ -398,6 +490,40 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
   size_t lds = sizeof(float2) * (sp.q.z + (size_t)sp.q.z * (P.lines + 1));
   (void)hipGetLastError();  // launch errors only, not a stale one
   hipLaunchKernelGGL(k_z, grid, dim3(kThreads), lds, stream, P);
+  return hipGetLastError();
+}
+
+bool z_grad_supported(const ShapePlan& sp) {
+  // two line sets of sp.bz lines in LDS
+  return !fast_size(sp.q.z) && sizeof(float2) * (sp.q.z + 2 * (size_t)sp.q.z * (sp.bz + 1)) <= 64 * 1024;
+}
+
+hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
+                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, hipStream_t stream) {
+  if (!z_grad_supported(sp) || zlo < 0 || zhi > sp.q.z || kz < 0 || kz > sp.q.z) return hipErrorInvalidValue;
+  ZGradParams P{};
+  P.spec = spec;
+  P.what = what;
+  P.khat = khat;
+  P.corr = corr;
+  P.mode = mode;
+  P.first = first;
+  P.shift = shift;
+  P.q = sp.q;
+  P.hx = sp.hx;
+  P.pitch = sp.pitch;
+  P.zlo = zlo;
+  P.zhi = zhi;
+  P.kz = kz;
+  P.lines = sp.bz;
+  P.plan = sp.pz;
+  P.tw = sp.tw_z;
+  P.twx = sp.tw_x;
+  P.twy = sp.tw_y;
+  dim3 grid((sp.hx + P.lines - 1) / P.lines, sp.q.y);
+  size_t lds = sizeof(float2) * (sp.q.z + 2 * (size_t)sp.q.z * (P.lines + 1));
+  (void)hipGetLastError();  // launch errors only, not a stale one
+  hipLaunchKernelGGL(k_z_grad, grid, dim3(kThreads), lds, stream, P);
   return hipGetLastError();
 }
 
@@ -500,6 +626,7 @@ int build_generic_params(const PassDesc& d, void* out, int* gx, int* gy, size_t*
     }
     case kPassZConv: {
       ZParams P{};
+      P.aux = nullptr;
       P.spec = d.spec;
       P.kspec = d.kspec;
       P.scale = d.scale;

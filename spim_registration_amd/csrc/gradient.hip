@@ -78,6 +78,25 @@ hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scal
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(kGThreads) void k_extract_psf_grad_shifted(const float* box, Int3 kd, float scale,
+                                                                       float* grad) {
+  const int64_t n = (int64_t)kd.x * kd.y * kd.z;
+  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads) {
+    const int x = (int)(i % kd.x), y = (int)((i / kd.x) % kd.y), z = (int)(i / ((int64_t)kd.x * kd.y));
+    grad[i] = box[(kd.x - 1 - x) + (int64_t)kd.x * ((kd.y - 1 - y) + (int64_t)kd.y * (kd.z - 1 - z))] * scale;
+  }
+}
+
+hipError_t launch_extract_psf_grad_shifted(const float* box, Int3 kd, float scale, float* grad, hipStream_t stream) {
+  const int64_t n = (int64_t)kd.x * kd.y * kd.z;
+  const int64_t blocks = std::min<int64_t>((n + kGThreads - 1) / kGThreads, 4096);
+  if (blocks == 0) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_extract_psf_grad_shifted, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, box, kd, scale,
+                     grad);
+  return hipGetLastError();
+}
+
 // dst (dense, dims size) = src[origin + (x, y, z)] of an image with dims n: the
 // residual of one block's effective region.
 __global__ __launch_bounds__(kGThreads) void k_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size) {

@@ -213,9 +213,18 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
 hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                     hipStream_t stream);
 // kspec == nullptr: one z transform in direction `dir`, then multiply by `scale`
-// (kernel spectra, gradient passes).
+// (kernel spectra, gradient passes).  aux (with kspec, generic lengths only):
+// also store the forward spectrum before the product, all rows.
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream, float dir = -1.f);
+                    hipStream_t stream, float dir = -1.f, float2* aux = nullptr);
+// Gradient z pass (generic lengths; z_grad_supported): forward R^ of the
+// y-transformed residual in spec; adjoint R^ conj(P^) inverted into rows
+// [zlo, zhi) of spec; correlation conj(R^) W^ with the lag shift `shift`
+// accumulated into corr (mode 0, first: overwrite) or inverted along z into
+// rows [0, kz) of corr (mode 1).
+bool z_grad_supported(const ShapePlan& sp);
+hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
+                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, hipStream_t stream);
 // Returns the number of workgroups (for the update statistics) in *nwg.
 hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                             hipStream_t stream, int* nwg);
@@ -344,6 +353,10 @@ hipError_t launch_spec_product(float2* out, const float2* a, const float2* b, in
 // out = conj(a) b when first, else out += conj(a) b.
 hipError_t launch_spec_corr_acc(float2* out, const float2* a, const float2* b, int first, int64_t n,
                                 hipStream_t stream);
+// grad[j] = scale * box[kd - 1 - j] per axis: the kernel-sized box of a
+// correlation whose lags were shifted by (kd - 1) / 2 (launch_z_grad), so that
+// lag kd / 2 - j lands on kd - 1 - j for odd and even kd.
+hipError_t launch_extract_psf_grad_shifted(const float* box, Int3 kd, float scale, float* grad, hipStream_t stream);
 // grad[j] = scale * box[(c - j) mod q] for the kernel box kd (x-fastest).
 hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, int accumulate,
                                    hipStream_t stream);

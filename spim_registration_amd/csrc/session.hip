@@ -1772,12 +1772,22 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
     MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
     MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nparts));
-    int rc = kernel_spectrum(s, sp, pk, k, &s->g_khat);
-    if (rc) return rc;
+    // the kernel spectrum: the prepared view's own when the PSF is its kernel1
+    const float2* Kh = nullptr;
+    if (s->prepared && view.plan == spp && view.k1hat.p && pk == view.k1) {
+      Kh = view.k1hat.as<float2>();
+    } else {
+      int rc = kernel_spectrum(s, sp, pk, k, &s->g_khat);
+      if (rc) return rc;
+      Kh = s->g_khat.as<float2>();
+    }
     float2* A = s->g_spec_a.as<float2>();
     float2* B = s->g_spec_b.as<float2>();
     float2* Cs = s->g_spec_c.as<float2>();
-    const float2* Kh = s->g_khat.as<float2>();
+    // z_grad: the residual's z pass forms both gradients' spectra in one
+    // launch; keep_w: one block, whose window spectrum W^ is kept from pass 1
+    const bool z_grad = z_grad_supported(sp);
+    const bool keep_w = z_grad && blocks.size() == 1;
     // pass 1, every block: b = E_mirror(psi) * P on the effective region, with the
     // residual r = w (1 - y / b) and the NLL partials w (b - y ln b) in the epilogue
     int off = 0;
@@ -1790,8 +1800,16 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       e.weight = wmode == MVD_WEIGHTS_NONE ? nullptr : s->d_weights.as<float>() + v * s->n;
       e.part_sum = s->g_part_sum.as<double>() + off;
       e.part_max = s->g_part_max.as<float>() + off;
-      rc = run_conv(s, sp, g, psrc, kGatherMirror, Kh, e, A);
-      if (rc) return rc;
+      if (keep_w) {
+        // W^ stored by the z pass before the product (into A), the convolution in B
+        const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
+        MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, B, st));
+        MVD_CHECK(launch_z(sp, B, Kh, 1.f, zlo, zhi, st, -1.f, A));
+        MVD_CHECK(launch_yx_inverse_passes(sp, g, B, e, st, nullptr));
+      } else {
+        int rc = run_conv(s, sp, g, psrc, kGatherMirror, Kh, e, A);
+        if (rc) return rc;
+      }
       off += conv_inverse_workgroups(sp, g);
     }
     MVD_CHECK(launch_reduce_stats(s->g_part_sum.as<double>(), s->g_part_max.as<float>(), off,
@@ -1800,26 +1818,77 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     if (!want_grad_psi && !want_psf) continue;
     const int64_t kn = (int64_t)k.x * k.y * k.z;
     if (want_psf) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kn));
+    const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
     // pass 2, every block, with the window spectrum recomputed instead of kept
     // (checkpointing: one block's spectra in memory at a time)
     for (size_t bi = 0; bi < blocks.size(); ++bi) {
       const ConvGeom& g = blocks[bi];
-      // W^ = FFT(E_mirror(psi) on the block window)
-      MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, A, st));
-      MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
-      // R^ = FFT(residual of this block's effective region, zero elsewhere in the window)
-      MVD_CHECK(launch_crop(s->g_crop.as<float>(), resid, n, g.eff_offset, g.eff_size, st));
+      if (!keep_w) {
+        // W^ = FFT(E_mirror(psi) on the block window)
+        MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, A, st));
+        MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
+      }
+      // R^ = FFT(residual of this block's effective region, zero elsewhere in the
+      // window); the residual image itself when that region is the whole image
+      const bool whole = g.eff_offset.x == 0 && g.eff_offset.y == 0 && g.eff_offset.z == 0 &&
+                         g.eff_size.x == n.x && g.eff_size.y == n.y && g.eff_size.z == n.z;
+      const float* rsrc = resid;
+      if (!whole) {
+        MVD_CHECK(launch_crop(s->g_crop.as<float>(), resid, n, g.eff_offset, g.eff_size, st));
+        rsrc = s->g_crop.as<float>();
+      }
       ConvGeom gc = g;
       gc.src = g.eff_size;
       gc.off = {-g.eff_local.x, -g.eff_local.y, -g.eff_local.z};
-      MVD_CHECK(launch_xy_forward_passes(sp, gc, s->g_crop.as<float>(), kGatherZero, Cs, st));
+      MVD_CHECK(launch_xy_forward_passes(sp, gc, rsrc, kGatherZero, Cs, st));
+      // the adjoint's output box: the block window
+      ConvGeom gw{};
+      gw.src = g.win;
+      gw.win = g.win;
+      gw.q = q;
+      gw.eff_size = g.win;
+      gw.dst = g.win;
+      const bool last = bi + 1 == blocks.size();
+      if (z_grad) {
+        // dL/dP(j) = sum over blocks of corr(R, W)(c - j): the lag-shifted
+        // correlation spectra are summed over the blocks (mode 0) and inverted
+        // once after the last, or, for one block, inverted along z in the same
+        // pass (mode 1); only the kernel-sized corner [0, kd) is transformed
+        // further.  The adjoint R^ conj(P^) is inverted along z into Cs.
+        float2* D = keep_w ? B : s->g_spec_d.as<float2>();
+        const Int3 shift = {(k.x - 1) / 2, (k.y - 1) / 2, (k.z - 1) / 2};
+        MVD_CHECK(launch_z_grad(sp, Cs, A, Kh, want_psf ? D : nullptr, keep_w ? 1 : 0, bi == 0, shift, 0,
+                                want_grad_psi ? g.win.z : 0, keep_w && want_psf ? k.z : 0, st));
+        if (want_psf && last) {
+          if (!keep_w) MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));
+          ConvGeom gk{};
+          gk.src = q;
+          gk.win = q;
+          gk.q = q;
+          gk.eff_size = k;
+          gk.dst = k;
+          EpilogueArgs es{};
+          es.mode = kEpiStore;
+          es.dst = s->g_box.as<float>();
+          MVD_CHECK(launch_yx_inverse_passes(sp, gk, D, es, st, nullptr));
+          MVD_CHECK(launch_extract_psf_grad_shifted(s->g_box.as<float>(), k, scale, s->g_kgrad.as<float>(), st));
+        }
+        if (want_grad_psi) {
+          EpilogueArgs es{};
+          es.mode = kEpiStore;
+          es.dst = s->g_box.as<float>();
+          MVD_CHECK(launch_yx_inverse_passes(sp, gw, Cs, es, st, nullptr));
+          MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
+        }
+        continue;
+      }
       MVD_CHECK(launch_z(sp, Cs, nullptr, 1.f, 0, q.z, st, -1.f));
       if (want_psf) {
         // dL/dP(j) = sum over blocks of corr(R, W)(c - j) = IFFT(sum conj(R^) W^)(c - j) / (qx qy qz):
         // the products are summed here and inverted once after the last block
         float2* D = blocks.size() > 1 ? s->g_spec_d.as<float2>() : B;
         MVD_CHECK(launch_spec_corr_acc(D, Cs, A, bi == 0, hs, st));
-        if (bi + 1 == blocks.size()) {
+        if (last) {
           MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));
           ConvGeom gb{};
           gb.src = q;
@@ -1831,7 +1900,6 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
           es.mode = kEpiStore;
           es.dst = s->g_box.as<float>();
           MVD_CHECK(launch_yx_inverse_passes(sp, gb, D, es, st, nullptr));
-          const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
           MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, s->g_kgrad.as<float>(), 0, st));
         }
       }
@@ -1839,16 +1907,10 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         // adjoint on the block window: IFFT(R^ conj(P^)), folded back through the mirror extension
         MVD_CHECK(launch_spec_product(B, Cs, Kh, 0, 1, hs, st));
         MVD_CHECK(launch_z(sp, B, nullptr, 1.f, 0, q.z, st, 1.f));
-        ConvGeom gb{};
-        gb.src = g.win;
-        gb.win = g.win;
-        gb.q = q;
-        gb.eff_size = g.win;
-        gb.dst = g.win;
         EpilogueArgs es{};
         es.mode = kEpiStore;
         es.dst = s->g_box.as<float>();
-        MVD_CHECK(launch_yx_inverse_passes(sp, gb, B, es, st, nullptr));
+        MVD_CHECK(launch_yx_inverse_passes(sp, gw, B, es, st, nullptr));
         MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
       }
     }

@@ -65,11 +65,14 @@ def test_explicit_psi_and_psfs_arguments():
     d.close()
 
 
-@pytest.mark.parametrize("block", [(32, 30, 24), (256, 256, 24)])
+@pytest.mark.parametrize("block", [(32, 30, 24), (256, 256, 24), (64, 64, 64)])
 def test_blocked_gradient_matches_oracle(block):
     """With blocks the gradient runs block by block (residual everywhere first,
     then per block a recomputed window spectrum); it must equal the whole-image
-    FP64 oracle.  (256, 256, 24) takes the fused plane kernels."""
+    FP64 oracle.  (256, 256, 24) takes the fused plane kernels; the 24-point
+    z passes the gradient z kernel, which sums the lag-shifted correlation
+    spectra over the blocks; (64, 64, 64) the register z kernels and the
+    separate spectrum products."""
     k = (9, 7, 11)
     ds = S.make_dataset((60, 50, 40), 3, [0, 120, 240], 40, k, 1.2, 3.0, seed=24)
     inp = MVDeconInput()
