@@ -224,6 +224,7 @@ struct ZGradParams {
   const float2* tw;
   const float2* twx;  // exp(-2 pi i m / q.x), m < q.x
   const float2* twy;  // exp(-2 pi i m / q.y), m < q.y
+  const double* rsum;  // R^(0) from the residual's double sum (nullable)
 };
 
 __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
@@ -245,6 +246,9 @@ __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
   __syncthreads();
   lds_fft(buf, lp, B, P.plan, tw, 1, -1.f);
   const bool adj = P.zhi > P.zlo, corr = P.corr != nullptr;
+  // the workgroup of line (kx, ky) = (0, 0) takes R^(0) from the double sum
+  const bool dc = P.rsum && c0 == 0 && y == 0;
+  const float dcv = dc ? (float)*gmem(P.rsum) : 0.f;
   // per-line part of the lag-shift phase (kx = c0 + b, ky = y)
   const int ty = (int)(((int64_t)y * P.shift.y) % P.q.y);
   const float2 py = gmem(P.twy)[ty];
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
     float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
     if (kx < P.hx) {
       const int64_t o = z * plane + col + b;
-      const float2 r = buf[z * lp + b];
+      const float2 r = dc && z == 0 && kx == 0 ? make_float2(dcv, 0.f) : buf[z * lp + b];
       if (adj) a = cmul(r, cconj(gmem(P.khat)[o]));
       if (corr) {
         const int tx = (int)(((int64_t)kx * P.shift.x) % P.q.x);
@@ -467,10 +471,7 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
 
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
                     hipStream_t stream, float dir, float2* aux) {
-  if (fast_size(sp.q.z)) {
-    if (aux) return hipErrorInvalidValue;
-    return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir);
-  }
+  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir, aux);
   ZParams P;
   P.aux = aux;
   P.dir = dir;
@@ -494,14 +495,18 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
 }
 
 bool z_grad_supported(const ShapePlan& sp) {
-  // two line sets of sp.bz lines in LDS
-  return !fast_size(sp.q.z) && sizeof(float2) * (sp.q.z + 2 * (size_t)sp.q.z * (sp.bz + 1)) <= 64 * 1024;
+  // generic lengths: two line sets of sp.bz lines in LDS
+  return fast_size(sp.q.z) || sizeof(float2) * (sp.q.z + 2 * (size_t)sp.q.z * (sp.bz + 1)) <= 64 * 1024;
 }
 
 hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
-                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, hipStream_t stream) {
+                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
+                         hipStream_t stream) {
   if (!z_grad_supported(sp) || zlo < 0 || zhi > sp.q.z || kz < 0 || kz > sp.q.z) return hipErrorInvalidValue;
+  if (fast_size(sp.q.z))
+    return launch_z_fast_grad(sp, spec, what, khat, corr, mode, first, shift, zlo, zhi, kz, rsum, stream);
   ZGradParams P{};
+  P.rsum = rsum;
   P.spec = spec;
   P.what = what;
   P.khat = khat;

@@ -1,7 +1,8 @@
-// Register-resident FFT passes for power-of-two lengths (64, 128, 256, 512):
+// Register-resident FFT passes for lengths 64, 128, 144, 256, 288 and 512:
 // the y pass, the fused z pass (forward, kernel product, inverse) and the
 // real-to-complex / complex-to-real x passes with their gather and epilogue.
 // The butterfly and LDS exchange building blocks live in fft_regs.h.
+#include <cstdlib>
 #include <cstring>
 
 #include "fft_regs.h"
@@ -23,12 +24,12 @@ struct YFastParams {
 template <int N>
 __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
   // lanes of the row padding [hx, pitch) transform zeros and store them, so
   // that every 128-byte line of a row is written whole
   const int col = bx * L + line;
@@ -58,12 +59,12 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 }
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_y_fast(YFastParams P) {
+__global__ __launch_bounds__(Spec<N>::NT) void k_y_fast(YFastParams P) {
   y_fast_body<N>(P, blockIdx.x, blockIdx.y);
 }
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_y_fast_batch(const YFastParams* items, const int* prefix, const int* gx,
+__global__ __launch_bounds__(Spec<N>::NT) void k_y_fast_batch(const YFastParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
@@ -87,17 +88,20 @@ struct ZFastParams {
   const unsigned short* ztab;
   const float2* arena;
   unsigned arena_bytes;
+  // with kspec (single launches only): the forward spectrum before the
+  // product, all rows (nullable)
+  float2* aux = nullptr;
 };
 
 template <int N>
 __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P.tw)[i];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
   const int col = bx * L + line;
   const bool valid = col < P.hx;
   // spectra hold fewer than 2^31 elements (lengths up to 1024): 32-bit offsets
@@ -129,6 +133,12 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
   }
   fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
   if (P.kspec) {
+    if (P.aux && valid) {
+#pragma unroll
+      for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+        for (int t = 0; t < RL; ++t) gmem(P.aux)[elem_index<N, E, RL>(jt, b, t) * plane + off] = v[b * RL + t];
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = cmul(v[i], kv[i]);
     fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
@@ -168,9 +178,9 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 // every table entry addresses a plane that no item of the launch overwrites,
 // so the items can run in any order.
 template <int N, bool TABLE>
-__global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
+__global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, L = kThreads / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
@@ -178,11 +188,11 @@ __global__ __launch_bounds__(kThreads, N == 512 ? 2 : 3) void k_z_fast_lines(con
   __shared__ unsigned short ztab[TABLE ? kZTabEntries : 1];
   const ZFastParams P0 = items[0];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = gmem(P0.tw)[i];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P0.tw)[i];
   if constexpr (TABLE) {
     for (int it = 0; it < nitems; ++it) {
       const unsigned short* src = items[it].ztab;
-      for (int i = threadIdx.x; i < N; i += kThreads) ztab[it * N + i] = gmem(src)[i];
+      for (int i = threadIdx.x; i < N; i += NT) ztab[it * N + i] = gmem(src)[i];
     }
     __syncthreads();
   }
@@ -258,14 +268,16 @@ hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, 
   if (table && nitems * n > kZTabEntries) return hipErrorInvalidValue;
   (void)hipGetLastError();
   const ZFastParams* it = static_cast<const ZFastParams*>(items);
-  const dim3 grid(gx, gy), block(kThreads);
-#define MVD_Z_LINES(NN)                                                                          \
-  if (table) hipLaunchKernelGGL((k_z_fast_lines<NN, true>), grid, block, 0, stream, it, nitems); \
-  else hipLaunchKernelGGL((k_z_fast_lines<NN, false>), grid, block, 0, stream, it, nitems);
+  const dim3 grid(gx, gy);
+#define MVD_Z_LINES(NN)                                                                                        \
+  if (table) hipLaunchKernelGGL((k_z_fast_lines<NN, true>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems); \
+  else hipLaunchKernelGGL((k_z_fast_lines<NN, false>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems);
   switch (n) {
     case 64: MVD_Z_LINES(64) break;
     case 128: MVD_Z_LINES(128) break;
+    case 144: MVD_Z_LINES(144) break;
     case 256: MVD_Z_LINES(256) break;
+    case 288: MVD_Z_LINES(288) break;
     case 512: MVD_Z_LINES(512) break;
     default: return hipErrorInvalidValue;
   }
@@ -274,12 +286,151 @@ hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, 
 }
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_z_fast(ZFastParams P) {
+__global__ __launch_bounds__(Spec<N>::NT) void k_z_fast(ZFastParams P) {
   z_fast_body<N>(P, blockIdx.x, blockIdx.y);
 }
 
+// z pass of the likelihood gradient on registers (see k_z_grad in
+// conv_kernels.hip for the generic lengths): forward transform of the
+// residual line R^; the correlation conj(R^) W^ times the lag-shift phase is
+// accumulated into corr (mode 0) or transformed back and rows [0, kz) stored
+// (mode 1); the adjoint R^ conj(P^) is transformed back and rows [zlo, zhi)
+// stored in place.
+struct ZGradFastParams {
+  float2* spec;
+  const float2* what;
+  const float2* khat;
+  float2* corr;
+  const float2* tw;
+  const float2* twx;
+  const float2* twy;
+  int mode, first;
+  Int3 shift, q;
+  int hx, pitch, zlo, zhi, kz;
+  const double* rsum;  // R^(0) from the residual's double sum (nullable)
+};
+
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_z_fast_batch(const ZFastParams* items, const int* prefix, const int* gx,
+__global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) {
+  using S = Spec<N>;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
+  constexpr int RL = last_radix<N>();
+  __shared__ float2 tw[N];
+  __shared__ float2 lds[N * LP];
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
+  const int col = blockIdx.x * L + line;
+  const int ky = blockIdx.y;
+  const bool valid = col < P.hx;
+  // spectra hold fewer than 2^31 elements (lengths up to 1024): 32-bit offsets
+  const int plane = P.q.y * P.pitch;
+  const int off = ky * P.pitch + col;
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) {
+      const int z = elem_index<N, E, S::R0>(jt, b, t);
+      v[b * S::R0 + t] = make_float2(0.f, 0.f);
+      if (valid) v[b * S::R0 + t] = gmem(P.spec)[z * plane + off];
+    }
+  auto addr = [line](int i) { return i * LP + line; };
+  fft_forward_order<N>(v, lds, jt, tw, 1, -1.f, addr);
+  // R^(0) (kz = 0: v[0] of thread jt = 0 of line (kx, ky) = (0, 0)) from the double sum
+  if (P.rsum && ky == 0 && col == 0 && jt == 0) v[0] = make_float2((float)*gmem(P.rsum), 0.f);
+  if (P.corr) {
+    // lag-shift phase exp(-2 pi i (kx sx / qx + ky sy / qy + kz sz / qz)): the
+    // (kx, ky) part is one factor per line
+    const int tx = (int)(((int64_t)(valid ? col : 0) * P.shift.x) % P.q.x);
+    const int ty = (int)(((int64_t)ky * P.shift.y) % P.q.y);
+    const float2 pl = cmul(gmem(P.twx)[tx], gmem(P.twy)[ty]);
+    float2 c[E];
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int z = elem_index<N, E, RL>(jt, b, t);
+        float2 w = make_float2(0.f, 0.f);
+        if (valid) w = gmem(P.what)[z * plane + off];
+        const float2 ph = cmul(pl, tw[(int)(((int64_t)z * P.shift.z) % N)]);
+        c[b * RL + t] = cmul(cmul(cconj(v[b * RL + t]), w), ph);
+      }
+    if (P.mode == 0) {
+      if (valid) {
+#pragma unroll
+        for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+          for (int t = 0; t < RL; ++t) {
+            const int o = elem_index<N, E, RL>(jt, b, t) * plane + off;
+            float2 x = c[b * RL + t];
+            if (!P.first) x = cadd(x, gmem(P.corr)[o]);
+            gmem(P.corr)[o] = x;
+          }
+      }
+    } else {
+      fft_reverse_order<N>(c, lds, jt, tw, 1, 1.f, addr);
+      if (valid) {
+#pragma unroll
+        for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+          for (int t = 0; t < S::R0; ++t) {
+            const int z = elem_index<N, E, S::R0>(jt, b, t);
+            if (z < P.kz) gmem(P.corr)[z * plane + off] = c[b * S::R0 + t];
+          }
+      }
+    }
+  }
+  if (P.zhi > P.zlo) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int z = elem_index<N, E, RL>(jt, b, t);
+        float2 p = make_float2(0.f, 0.f);
+        if (valid) p = gmem(P.khat)[z * plane + off];
+        v[b * RL + t] = cmul(v[b * RL + t], cconj(p));
+      }
+    fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
+    if (valid) {
+#pragma unroll
+      for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+        for (int t = 0; t < S::R0; ++t) {
+          const int z = elem_index<N, E, S::R0>(jt, b, t);
+          if (z >= P.zlo && z < P.zhi) gmem(P.spec)[z * plane + off] = v[b * S::R0 + t];
+        }
+    }
+  }
+}
+
+hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
+                              int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
+                              hipStream_t stream) {
+  ZGradFastParams P{spec, what, khat, corr, sp.tw_z, sp.tw_x, sp.tw_y, mode, first, shift, sp.q,
+                    sp.hx, sp.pitch, zlo, zhi, kz, rsum};
+  (void)hipGetLastError();
+#define MVD_ZG(N)                                                                               \
+  case N: {                                                                                     \
+    constexpr int L = Spec<N>::NT / (N / Spec<N>::E);                                           \
+    dim3 grid((sp.hx + L - 1) / L, sp.q.y);                                                     \
+    hipLaunchKernelGGL(k_z_fast_grad<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                \
+    break;                                                                                      \
+  }
+  switch (sp.q.z) {
+    MVD_ZG(64)
+    MVD_ZG(128)
+    MVD_ZG(144)
+    MVD_ZG(256)
+    MVD_ZG(288)
+    MVD_ZG(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef MVD_ZG
+  return hipGetLastError();
+}
+
+template <int N>
+__global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_batch(const ZFastParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
@@ -306,20 +457,23 @@ template <int M, int MODE>
 __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int bx, int by) {
   static_assert(M % 16 == 0, "the row pitch is M + 16");
   using S = Spec<M>;
-  constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  constexpr int E = S::E, T = M / E, NT = S::NT, L = NT / T, MP = M + 1;
   constexpr int W = 2 * M;                                   // real samples per row
-  constexpr int RPI = W >= kThreads ? 1 : kThreads / W;      // rows per loop trip
-  constexpr int EPT = W >= kThreads ? W / kThreads : 1;      // samples per thread per row
+  constexpr int RPI = W >= NT ? 1 : NT / W;                  // rows per loop trip
+  constexpr int EPT = W >= NT ? (W + NT - 1) / NT : 1;       // samples per thread per row
+  static_assert(W >= NT || NT % W == 0, "rows per trip");
   __shared__ float2 tw[2 * M];
   __shared__ float2 buf[L * MP];
   float* fbuf = reinterpret_cast<float*>(buf);
   const ConvGeom& g = P.g;
-  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = gmem(P.tw)[i];
+  for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = gmem(P.tw)[i];
   const int nrows = g.q.y * g.q.z;
   const int row0 = bx * L;
   const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
   const int lsub = RPI > 1 ? threadIdx.x / W : 0;
   constexpr int NR = L / RPI;  // rows handled by each thread
+  // the last trip over a row is partial when NT does not divide W
+  auto x_in_row = [&](int e) { return W % NT == 0 || x0 + e * NT < W; };
   // all loads of the thread first (independent, many in flight), then LDS stores
   float vals[NR][EPT];
 #pragma unroll
@@ -350,7 +504,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
     }
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int x = x0 + e * kThreads;
+      const int x = x0 + e * NT;
       float val = 0.f;
       if (live && x < g.win.x) {
         if (MODE == kGatherKernelWrap) {
@@ -376,7 +530,8 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
 #pragma unroll
   for (int r = 0; r < NR; ++r)
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) fbuf[(r * RPI + lsub) * 2 * MP + x0 + e * kThreads] = vals[r][e];
+    for (int e = 0; e < EPT; ++e)
+      if (x_in_row(e)) fbuf[(r * RPI + lsub) * 2 * MP + x0 + e * NT] = vals[r][e];
   __syncthreads();
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   auto addr = [line](int i) { return line * MP + i; };
@@ -397,11 +552,11 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
   // so that the row's last 128-byte line is written by one store (a partly
   // written line reaches HBM as a read-modify-write)
   constexpr int K2 = M / 2 + 1;
-  for (int idx = threadIdx.x; idx < L * K2; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < L * K2; idx += NT) {
     const int l = idx / K2, k = idx % K2;
     const int row = row0 + l;
     if (row >= nrows) continue;
-    const float2 a = buf[l * MP + k], b = buf[l * MP + ((M - k) & (M - 1))];
+    const float2 a = buf[l * MP + k], b = buf[l * MP + (M - k) % M];
     const auto out = gmem(P.spec) + (int64_t)row * P.pitch;
     {
       const float2 bc = cconj(b);
@@ -414,7 +569,7 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
       stream_store(&out[M - k], cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x))));
     }
   }
-  for (int idx = threadIdx.x; idx < L * 16; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < L * 16; idx += NT) {
     const int l = idx >> 4, c = idx & 15;
     const int row = row0 + l;
     if (row >= nrows) continue;
@@ -430,12 +585,12 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
 }
 
 template <int M, int MODE>
-__global__ __launch_bounds__(kThreads) void k_x_forward_fast(XFwdFastParams P) {
+__global__ __launch_bounds__(Spec<M>::NT) void k_x_forward_fast(XFwdFastParams P) {
   x_forward_fast_body<M, MODE>(P, blockIdx.x, blockIdx.y);
 }
 
 template <int M, int MODE>
-__global__ __launch_bounds__(kThreads) void k_x_forward_fast_batch(const XFwdFastParams* items, const int* prefix, const int* gx,
+__global__ __launch_bounds__(Spec<M>::NT) void k_x_forward_fast_batch(const XFwdFastParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
@@ -454,18 +609,22 @@ struct XInvFastParams {
 template <int M, int EPI>
 __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int bx, int by) {
   using S = Spec<M>;
-  constexpr int E = S::E, T = M / E, L = kThreads / T, MP = M + 1;
+  constexpr int E = S::E, T = M / E, NT = S::NT, L = NT / T, MP = M + 1;
   constexpr int W = 2 * M;
-  constexpr int RPI_C = M >= kThreads ? 1 : kThreads / M;   // rows per trip when loading M complex
-  constexpr int EPT_C = M >= kThreads ? M / kThreads : 1;
-  constexpr int RPI = W >= kThreads ? 1 : kThreads / W;     // rows per trip over 2M reals
-  constexpr int EPT = W >= kThreads ? W / kThreads : 1;
+  // rows per trip and values per thread when loading M complex per row: a
+  // row spans whole trips (NT | M), or a trip holds whole rows (M | NT), or
+  // one row per trip with the lanes past M idle
+  constexpr int RPI_C = M >= NT || NT % M ? 1 : NT / M;
+  constexpr int EPT_C = M >= NT ? (M + NT - 1) / NT : 1;
+  constexpr int RPI = W >= NT ? 1 : NT / W;     // rows per trip over 2M reals
+  constexpr int EPT = W >= NT ? (W + NT - 1) / NT : 1;
+  static_assert(W >= NT || NT % W == 0, "rows per trip");
   __shared__ float2 tw[2 * M];
   __shared__ float2 buf[L * MP];
-  __shared__ double red_sum[kThreads / 64];
-  __shared__ float red_max[kThreads / 64];
+  __shared__ double red_sum[NT / 64];
+  __shared__ float red_max[NT / 64];
   const ConvGeom& g = P.g;
-  for (int i = threadIdx.x; i < 2 * M; i += kThreads) tw[i] = gmem(P.tw)[i];
+  for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = gmem(P.tw)[i];
   const int ny = g.eff_size.y;
   const int nrows = ny * g.eff_size.z;
   const int row0 = bx * L;
@@ -485,15 +644,16 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
 #pragma unroll
       for (int e = 0; e < EPT_C; ++e) {
         vals[r][e] = make_float2(0.f, 0.f);
-        if (srow) vals[r][e] = stream_load(&srow[k0 + e * kThreads]);
+        if (srow && k0 + e * NT < M) vals[r][e] = stream_load(&srow[k0 + e * NT]);
       }
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-      for (int e = 0; e < EPT_C; ++e) buf[(r * RPI_C + lsub) * MP + k0 + e * kThreads] = vals[r][e];
+      for (int e = 0; e < EPT_C; ++e)
+        if (k0 + e * NT < M) buf[(r * RPI_C + lsub) * MP + k0 + e * NT] = vals[r][e];
     // the Nyquist column k = M
-    for (int l = threadIdx.x; l < L; l += kThreads) {
+    for (int l = threadIdx.x; l < L; l += NT) {
       const int row = row0 + l;
       float2 val = make_float2(0.f, 0.f);
       if (row < nrows) {
@@ -505,7 +665,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   }
   __syncthreads();
   constexpr int K2 = M / 2 + 1;
-  for (int idx = threadIdx.x; idx < L * K2; idx += kThreads) {
+  for (int idx = threadIdx.x; idx < L * K2; idx += NT) {
     const int l = idx / K2, k = idx % K2;
     const float2 a = buf[l * MP + k], b = buf[l * MP + M - k];
     const float2 za = cadd(cadd(a, cconj(b)), mul_di(cmul(csub(a, cconj(b)), cconj(tw[k])), 1.f));
@@ -550,7 +710,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     const int base = g.eff_offset.x + g.dst.x * ((g.eff_offset.y + yo) + g.dst.y * (g.eff_offset.z + zo));
 #pragma unroll
     for (int ee = 0; ee < EPT; ++ee) {
-      const int xo = x0 + ee * kThreads;
+      const int xo = x0 + ee * NT;
       const bool live = live_row && xo < nx;
       const int gi = live ? base + xo : -1;
       gidx[r][ee] = gi;
@@ -575,7 +735,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     for (int ee = 0; ee < EPT; ++ee) {
       const int gi = gidx[r][ee];
       if (gi < 0) continue;
-      const int xo = x0 + ee * kThreads;
+      const int xo = x0 + ee * NT;
       const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
       if (EPI == kEpiStore) {
         gmem(e.dst)[gi] = val;
@@ -618,7 +778,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     if (threadIdx.x == 0) {
       double s = 0.0;
       float m = -1.f;
-      for (int i = 0; i < kThreads / 64; ++i) {
+      for (int i = 0; i < NT / 64; ++i) {
         s += red_sum[i];
         m = java_maxf(m, red_max[i]);
       }
@@ -629,12 +789,12 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
 }
 
 template <int M, int EPI>
-__global__ __launch_bounds__(kThreads) void k_x_inverse_fast(XInvFastParams P) {
+__global__ __launch_bounds__(Spec<M>::NT) void k_x_inverse_fast(XInvFastParams P) {
   x_inverse_fast_body<M, EPI>(P, blockIdx.x, blockIdx.y);
 }
 
 template <int M, int EPI>
-__global__ __launch_bounds__(kThreads) void k_x_inverse_fast_batch(const XInvFastParams* items, const int* prefix, const int* gx,
+__global__ __launch_bounds__(Spec<M>::NT) void k_x_inverse_fast_batch(const XInvFastParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
   int item, bx, by;
   batch_locate(prefix, gx, nitems, item, bx, by);
@@ -647,19 +807,39 @@ __global__ __launch_bounds__(kThreads) void k_x_inverse_fast_batch(const XInvFas
 
 template <int N>
 static constexpr int lines_per_wg() {
-  return kThreads / (N / Spec<N>::E);
+  return Spec<N>::NT / (N / Spec<N>::E);
 }
 
-bool fast_size(int n) { return n == 64 || n == 128 || n == 256 || n == 512; }
+// MVD_FAST_SMOOTH=0 keeps the 144- and 288-point lengths on the generic LDS
+// kernels (comparisons).
+static const bool g_fast_smooth = [] {
+  const char* v = std::getenv("MVD_FAST_SMOOTH");
+  return !(v && v[0] == '0');
+}();
+
+bool fast_size(int n) {
+  return n == 64 || n == 128 || n == 256 || n == 512 || (g_fast_smooth && (n == 144 || n == 288));
+}
 
 int x_fast_rows_per_wg(int M) {
   switch (M) {
     case 64: return lines_per_wg<64>();
     case 128: return lines_per_wg<128>();
+    case 144: return lines_per_wg<144>();
     case 256: return lines_per_wg<256>();
+    case 288: return lines_per_wg<288>();
     case 512: return lines_per_wg<512>();
   }
   return 0;
+}
+
+// threads per workgroup of the kernels of length n
+static int fast_threads(int n) {
+  switch (n) {
+    case 144: return Spec<144>::NT;
+    case 288: return Spec<288>::NT;
+  }
+  return kThreads;
 }
 
 bool pass_is_fast(const PassDesc& d) {
@@ -682,7 +862,9 @@ static int fast_lines(int n) {
   switch (n) {
     case 64: return lines_of<64>();
     case 128: return lines_of<128>();
+    case 144: return lines_of<144>();
     case 256: return lines_of<256>();
+    case 288: return lines_of<288>();
     case 512: return lines_of<512>();
   }
   return 1;
@@ -738,7 +920,10 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
                              int nwg, hipStream_t stream) {
   if (nwg <= 0) return hipSuccess;
   (void)hipGetLastError();
-  const dim3 grid(nwg), block(kThreads);
+  const dim3 grid(nwg);
+  // the class encodes the length: y and z passes cls = N, x passes cls = 16 M + mode
+  const int n = kind == kPassXForward || kind == kPassXInverse ? cls / 16 : cls;
+  const dim3 block(fast_threads(n));
   switch (kind) {
     case kPassYForward:
     case kPassYInverse: {
@@ -746,7 +931,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
       switch (cls) {
         case 64: hipLaunchKernelGGL(k_y_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 128: hipLaunchKernelGGL(k_y_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 144: hipLaunchKernelGGL(k_y_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 256: hipLaunchKernelGGL(k_y_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 288: hipLaunchKernelGGL(k_y_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 512: hipLaunchKernelGGL(k_y_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         default: return hipErrorInvalidValue;
       }
@@ -757,7 +944,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
       switch (cls) {
         case 64: hipLaunchKernelGGL(k_z_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 128: hipLaunchKernelGGL(k_z_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 144: hipLaunchKernelGGL(k_z_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 256: hipLaunchKernelGGL(k_z_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 288: hipLaunchKernelGGL(k_z_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 512: hipLaunchKernelGGL(k_z_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         default: return hipErrorInvalidValue;
       }
@@ -773,7 +962,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
       switch (cls) {
         MVD_BXF4(64)
         MVD_BXF4(128)
+        MVD_BXF4(144)
         MVD_BXF4(256)
+        MVD_BXF4(288)
         MVD_BXF4(512)
         default: return hipErrorInvalidValue;
       }
@@ -791,7 +982,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
       switch (cls) {
         MVD_BXI4(64)
         MVD_BXI4(128)
+        MVD_BXI4(144)
         MVD_BXI4(256)
+        MVD_BXI4(288)
         MVD_BXI4(512)
         default: return hipErrorInvalidValue;
       }
@@ -813,13 +1006,15 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
 #define MVD_Y(N)                                                                                \
   case N: {                                                                                     \
     dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), zhi - zlo);                  \
-    hipLaunchKernelGGL(k_y_fast<N>, grid, dim3(kThreads), 0, stream, P);                        \
+    hipLaunchKernelGGL(k_y_fast<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                     \
     break;                                                                                      \
   }
   switch (n) {
     MVD_Y(64)
     MVD_Y(128)
+    MVD_Y(144)
     MVD_Y(256)
+    MVD_Y(288)
     MVD_Y(512)
     default: return hipErrorInvalidValue;
   }
@@ -828,19 +1023,21 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
 }
 
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream, float dir) {
-  ZFastParams P{spec, kspec, sp.tw_z, scale, dir, sp.hx, sp.pitch, sp.q.y, zlo, zhi};
+                         hipStream_t stream, float dir, float2* aux) {
+  ZFastParams P{spec, kspec, sp.tw_z, scale, dir, sp.hx, sp.pitch, sp.q.y, zlo, zhi, nullptr, nullptr, 0, aux};
   (void)hipGetLastError();
 #define MVD_Z(N)                                                                                \
   case N: {                                                                                     \
     dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), sp.q.y);                     \
-    hipLaunchKernelGGL(k_z_fast<N>, grid, dim3(kThreads), 0, stream, P);                        \
+    hipLaunchKernelGGL(k_z_fast<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                     \
     break;                                                                                      \
   }
   switch (sp.q.z) {
     MVD_Z(64)
     MVD_Z(128)
+    MVD_Z(144)
     MVD_Z(256)
+    MVD_Z(288)
     MVD_Z(512)
     default: return hipErrorInvalidValue;
   }
@@ -856,7 +1053,7 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
   (void)hipGetLastError();
 #define MVD_XF_MODE(N, MODE)                                                                    \
   case MODE:                                                                                    \
-    hipLaunchKernelGGL((k_x_forward_fast<N, MODE>), grid, dim3(kThreads), 0, stream, P);        \
+    hipLaunchKernelGGL((k_x_forward_fast<N, MODE>), grid, dim3(Spec<N>::NT), 0, stream, P);     \
     break;
 #define MVD_XF(N)                                                                               \
   case N: {                                                                                     \
@@ -873,7 +1070,9 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
   switch (M) {
     MVD_XF(64)
     MVD_XF(128)
+    MVD_XF(144)
     MVD_XF(256)
+    MVD_XF(288)
     MVD_XF(512)
     default: return hipErrorInvalidValue;
   }
@@ -889,7 +1088,7 @@ hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const f
   (void)hipGetLastError();
 #define MVD_XI_EPI(N, EPI)                                                                      \
   case EPI:                                                                                     \
-    hipLaunchKernelGGL((k_x_inverse_fast<N, EPI>), dim3(nwg), dim3(kThreads), 0, stream, P);    \
+    hipLaunchKernelGGL((k_x_inverse_fast<N, EPI>), dim3(nwg), dim3(Spec<N>::NT), 0, stream, P); \
     break;
 #define MVD_XI(N)                                                                               \
   case N:                                                                                       \
@@ -904,7 +1103,9 @@ hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const f
   switch (M) {
     MVD_XI(64)
     MVD_XI(128)
+    MVD_XI(144)
     MVD_XI(256)
+    MVD_XI(288)
     MVD_XI(512)
     default: return hipErrorInvalidValue;
   }

@@ -3,10 +3,11 @@
 //
 // Every line is transformed by T = N / E cooperating threads that each keep
 // E complex values in VGPRs.  A length is split into two or three radices
-// (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8); butterflies run
-// in registers and the lines meet in LDS only between radix passes -- one LDS
-// exchange for a 256-point line instead of one per radix-2/4 stage.  All
-// sizes are compile-time, so index arithmetic is shifts and masks.
+// (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8, 288 = 24 x 12,
+// 144 = 12 x 12); butterflies run in registers and the lines meet in LDS only
+// between radix passes -- one LDS exchange for a 256-point line instead of one
+// per radix-2/4 stage.  All sizes are compile-time, so index arithmetic is
+// shifts, masks and multiplications by constants.
 //
 // Stockham decimation in time: in a pass of radix R with Ns = product of the
 // earlier radices, butterfly j reads x[j + t N/R], multiplies by
@@ -62,23 +63,88 @@ __device__ __forceinline__ void butterfly<16>(float2* v, float dir) {
   }
 }
 
+// DFT of size N = P Q on v[0, N): n = n1 + P n2, k = k1 + Q k2; inner DFTs of
+// size Q over n2, twiddles W_N^(n1 k1), outer DFTs of size P over n1.
+// cs[m], sn[m]: cos and sin of 2 pi m / N.
+template <int P, int Q>
+__device__ __forceinline__ void butterfly_pq(float2* v, float dir, const float* cs, const float* sn) {
+  constexpr int N = P * Q;
+  float2 a[P][Q];
+#pragma unroll
+  for (int n1 = 0; n1 < P; ++n1) {
+    float2 t[Q];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) t[n2] = v[n1 + P * n2];
+    butterfly<Q>(t, dir);
+#pragma unroll
+    for (int k1 = 0; k1 < Q; ++k1) a[n1][k1] = t[k1];
+  }
+#pragma unroll
+  for (int n1 = 1; n1 < P; ++n1)
+#pragma unroll
+    for (int k1 = 1; k1 < Q; ++k1) {
+      const int m = (n1 * k1) % N;
+      a[n1][k1] = cmul(a[n1][k1], make_float2(cs[m], dir * sn[m]));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < Q; ++k1) {
+    float2 t[P];
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) t[n1] = a[n1][k1];
+    butterfly<P>(t, dir);
+#pragma unroll
+    for (int k2 = 0; k2 < P; ++k2) v[k1 + Q * k2] = t[k2];
+  }
+}
+
+template <>
+__device__ __forceinline__ void butterfly<12>(float2* v, float dir) {
+  const float r3 = 0.866025403784438646763723f;  // sqrt(3) / 2
+  const float cs[12] = {1.f, r3, 0.5f, 0.f, -0.5f, -r3, -1.f, -r3, -0.5f, 0.f, 0.5f, r3};
+  const float sn[12] = {0.f, 0.5f, r3, 1.f, r3, 0.5f, 0.f, -0.5f, -r3, -1.f, -r3, -0.5f};
+  butterfly_pq<4, 3>(v, dir, cs, sn);
+}
+
+template <>
+__device__ __forceinline__ void butterfly<24>(float2* v, float dir) {
+  const float c1 = 0.965925826289068286749743f;  // cos(pi / 12)
+  const float s1 = 0.258819045102520762348899f;  // sin(pi / 12)
+  const float r3 = 0.866025403784438646763723f;
+  const float h = 0.707106781186547524400844f;
+  const float cs[24] = {1.f, c1, r3, h, 0.5f, s1, 0.f, -s1, -0.5f, -h, -r3, -c1,
+                        -1.f, -c1, -r3, -h, -0.5f, -s1, 0.f, s1, 0.5f, h, r3, c1};
+  const float sn[24] = {0.f, s1, 0.5f, h, r3, c1, 1.f, c1, r3, h, 0.5f, s1,
+                        0.f, -s1, -0.5f, -h, -r3, -c1, -1.f, -c1, -r3, -h, -0.5f, -s1};
+  butterfly_pq<8, 3>(v, dir, cs, sn);
+}
+
+// E values per thread, radices R0 x R1 (x R2), NT threads per workgroup (one
+// workgroup holds NT / (N / E) lines).
 template <int N>
 struct Spec;
 template <>
 struct Spec<64> {
-  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1;
+  static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1, NT = kThreads;
 };
 template <>
 struct Spec<128> {
-  static constexpr int E = 16, R0 = 16, R1 = 8, R2 = 1;
+  static constexpr int E = 16, R0 = 16, R1 = 8, R2 = 1, NT = kThreads;
+};
+template <>
+struct Spec<144> {
+  static constexpr int E = 12, R0 = 12, R1 = 12, R2 = 1, NT = 192;
 };
 template <>
 struct Spec<256> {
-  static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1;
+  static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1, NT = kThreads;
+};
+template <>
+struct Spec<288> {
+  static constexpr int E = 24, R0 = 24, R1 = 12, R2 = 1, NT = 192;
 };
 template <>
 struct Spec<512> {
-  static constexpr int E = 16, R0 = 8, R1 = 8, R2 = 8;
+  static constexpr int E = 16, R0 = 8, R1 = 8, R2 = 8, NT = kThreads;
 };
 
 // Butterflies of one pass on the registers (inputs already in pass order).
@@ -89,7 +155,7 @@ __device__ __forceinline__ void bfly_pass(float2 (&v)[E], int jt, const float2* 
 #pragma unroll
   for (int b = 0; b < E / R; ++b) {
     if constexpr (NS > 1) {
-      const int k = (jt + T * b) & (NS - 1);
+      const int k = (jt + T * b) % NS;
 #pragma unroll
       for (int t = 1; t < R; ++t) {
         float2 w = tw[t * k * TS * tw_step];
@@ -107,7 +173,7 @@ __device__ __forceinline__ void store_pass(const float2 (&v)[E], float2* lds, in
 #pragma unroll
   for (int b = 0; b < E / R; ++b) {
     const int j = jt + T * b;
-    const int base = (j / NS) * NS * R + (j & (NS - 1));
+    const int base = (j / NS) * NS * R + j % NS;
 #pragma unroll
     for (int t = 0; t < R; ++t) lds[addr(base + t * NS)] = v[b * R + t];
   }

@@ -199,6 +199,37 @@ hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double
   return hipGetLastError();
 }
 
+// out = sum of the partials part[0, n) in double (one workgroup, a fixed order).
+__global__ __launch_bounds__(kGThreads) void k_reduce_sum(const double* part, int n, double* out) {
+  __shared__ double rs[kGThreads];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kGThreads) v += part[i];
+  rs[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kGThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) rs[threadIdx.x] += rs[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = rs[0];
+}
+
+hipError_t launch_sum_to(const float* a, int64_t n, double* part, int nparts, double* out, hipStream_t stream) {
+  hipError_t e = launch_sum(a, n, part, nparts, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kGThreads), 0, stream, part, nparts, out);
+  return hipGetLastError();
+}
+
+// spec[0] = (float) *v: the zero-frequency value of a real line set from its
+// double sum.
+__global__ void k_set_dc(float2* spec, const double* v) { spec[0] = make_float2((float)*v, 0.f); }
+
+hipError_t launch_set_dc(float2* spec, const double* v, hipStream_t stream) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_set_dc, dim3(1), dim3(1), 0, stream, spec, v);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kGThreads) void k_to_float(const double* src, float* dst, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads)
     dst[i] = (float)src[i];

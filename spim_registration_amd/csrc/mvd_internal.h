@@ -213,18 +213,22 @@ hipError_t launch_x_forward(const ShapePlan& sp, const ConvGeom& g, const float*
 hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                     hipStream_t stream);
 // kspec == nullptr: one z transform in direction `dir`, then multiply by `scale`
-// (kernel spectra, gradient passes).  aux (with kspec, generic lengths only):
-// also store the forward spectrum before the product, all rows.
+// (kernel spectra, gradient passes).  aux (with kspec): also store the forward
+// spectrum before the product, all rows.
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
                     hipStream_t stream, float dir = -1.f, float2* aux = nullptr);
-// Gradient z pass (generic lengths; z_grad_supported): forward R^ of the
+// Gradient z pass (z_grad_supported): forward R^ of the
 // y-transformed residual in spec; adjoint R^ conj(P^) inverted into rows
 // [zlo, zhi) of spec; correlation conj(R^) W^ with the lag shift `shift`
 // accumulated into corr (mode 0, first: overwrite) or inverted along z into
-// rows [0, kz) of corr (mode 1).
+// rows [0, kz) of corr (mode 1).  rsum (nullable): the residual's sum in
+// double, which replaces the zero-frequency value R^(0) of the transform (an
+// FP32 transform of a residual of mixed signs loses that value's low bits,
+// and the correlation multiplies it by the window's sum).
 bool z_grad_supported(const ShapePlan& sp);
 hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
-                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, hipStream_t stream);
+                         int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
+                         hipStream_t stream);
 // Returns the number of workgroups (for the update statistics) in *nwg.
 hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                             hipStream_t stream, int* nwg);
@@ -272,7 +276,10 @@ int x_fast_rows_per_wg(int M);
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                          hipStream_t stream);
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream, float dir);
+                         hipStream_t stream, float dir, float2* aux = nullptr);
+hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
+                              int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
+                              hipStream_t stream);
 hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                                  hipStream_t stream);
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
@@ -364,6 +371,10 @@ hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 s
 hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream);
 // acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
 hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
+// *out = sum of a[0, n) in double, through nparts partials in part.
+hipError_t launch_sum_to(const float* a, int64_t n, double* part, int nparts, double* out, hipStream_t stream);
+// spec[0] = (float) *v (the zero-frequency value of a spectrum).
+hipError_t launch_set_dc(float2* spec, const double* v, hipStream_t stream);
 hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream);
 
 // Resampling of one view into the bounding box with its blending weight

@@ -412,6 +412,18 @@ static const bool g_plane_dedup = [] {
   return !(v && v[0] == '0');
 }();
 
+// MVD_GRAD_FUSED=0: the gradient's spectrum products and z transforms as
+// separate launches; MVD_GRAD_KEEP_W=0: the window spectrum recomputed in the
+// second pass also for one block (comparisons).
+static const bool g_grad_fused = [] {
+  const char* v = std::getenv("MVD_GRAD_FUSED");
+  return !(v && v[0] == '0');
+}();
+static const bool g_grad_keep_w = [] {
+  const char* v = std::getenv("MVD_GRAD_KEEP_W");
+  return !(v && v[0] == '0');
+}();
+
 static int mirror_single_host(int p, int n) {
   if (p >= 0 && p < n) return p;
   if (n == 1) return 0;
@@ -621,6 +633,7 @@ struct mvd_session {
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
   DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
+  DeviceBuf g_rsum;  // partials and the double sum of one block's residual
   DeviceBuf g_crop, g_nll, g_gpsi;
   bool g_gpsi_ready = false;
   PlanCache plans;
@@ -969,10 +982,20 @@ int kernel_spectrum(mvd_session* s, const ShapePlan& sp, const std::vector<float
   T.begin(s->stream);
   MVD_CHECK(launch_y(sp, out->as<float2>(), 0, sp.q.z, 0, sp.q.y, -1.f, s->stream));
   T.end(s->stream, kTYForward);
-  const float scale = (float)(1.0L / ((long double)sp.q.x * sp.q.y * sp.q.z));
+  const long double n = (long double)sp.q.x * sp.q.y * sp.q.z;
+  const float scale = (float)(1.0L / n);
   T.begin(s->stream);
   MVD_CHECK(launch_z(sp, out->as<float2>(), nullptr, scale, 0, sp.q.z, s->stream));
   T.end(s->stream, kTZForward);
+  // The zero-frequency value is the kernel's sum over N: set from the sum in
+  // long double, rounded once.  Its FP32 transform can be an ulp or two off,
+  // which scales every convolution with the kernel by as much; the likelihood
+  // gradient in the PSF responds to that scale about a thousand times more
+  // strongly than to the PSF values themselves near a fitted estimate.
+  long double sum = 0.0L;
+  for (float v : k) sum += v;
+  const float2 dc = make_float2((float)(sum / n), 0.f);
+  MVD_CHECK(hipMemcpyAsync(out->p, &dc, sizeof(dc), hipMemcpyHostToDevice, s->stream));
   MVD_CHECK(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -1767,11 +1790,21 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_spec_a.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_b.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_spec_c.ensure(sizeof(float2) * hs));
-    if (grad_psf && grad_psf[v] && blocks.size() > 1) MVD_CHECK(s->g_spec_d.ensure(sizeof(float2) * hs));
+    // z_grad: the residual's z pass forms both gradients' spectra in one
+    // launch; keep_w: one block, whose window spectrum W^ is kept from pass 1
+    const bool z_grad = g_grad_fused && z_grad_supported(sp);
+    const bool keep_w = z_grad && g_grad_keep_w && blocks.size() == 1;
+    // the summed correlation spectrum D: its own buffer unless one block's
+    // correlation goes through B (keep_w, or the separate launches with one block)
+    const bool own_d = z_grad ? !keep_w : blocks.size() > 1;
+    if (grad_psf && grad_psf[v] && own_d) MVD_CHECK(s->g_spec_d.ensure(sizeof(float2) * hs));
     MVD_CHECK(s->g_box.ensure(sizeof(float) * box));
     MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
     MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
     MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nparts));
+    constexpr int kRsumParts = 256;
+    MVD_CHECK(s->g_rsum.ensure(sizeof(double) * (kRsumParts + 1)));
+    double* rsum = s->g_rsum.as<double>() + kRsumParts;
     // the kernel spectrum: the prepared view's own when the PSF is its kernel1
     const float2* Kh = nullptr;
     if (s->prepared && view.plan == spp && view.k1hat.p && pk == view.k1) {
@@ -1784,10 +1817,6 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     float2* A = s->g_spec_a.as<float2>();
     float2* B = s->g_spec_b.as<float2>();
     float2* Cs = s->g_spec_c.as<float2>();
-    // z_grad: the residual's z pass forms both gradients' spectra in one
-    // launch; keep_w: one block, whose window spectrum W^ is kept from pass 1
-    const bool z_grad = z_grad_supported(sp);
-    const bool keep_w = z_grad && blocks.size() == 1;
     // pass 1, every block: b = E_mirror(psi) * P on the effective region, with the
     // residual r = w (1 - y / b) and the NLL partials w (b - y ln b) in the epilogue
     int off = 0;
@@ -1841,6 +1870,9 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       gc.src = g.eff_size;
       gc.off = {-g.eff_local.x, -g.eff_local.y, -g.eff_local.z};
       MVD_CHECK(launch_xy_forward_passes(sp, gc, rsrc, kGatherZero, Cs, st));
+      // R^(0) = the residual's sum, in double
+      MVD_CHECK(launch_sum_to(rsrc, (int64_t)g.eff_size.x * g.eff_size.y * g.eff_size.z, s->g_rsum.as<double>(),
+                              kRsumParts, rsum, st));
       // the adjoint's output box: the block window
       ConvGeom gw{};
       gw.src = g.win;
@@ -1855,10 +1887,11 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         // once after the last, or, for one block, inverted along z in the same
         // pass (mode 1); only the kernel-sized corner [0, kd) is transformed
         // further.  The adjoint R^ conj(P^) is inverted along z into Cs.
-        float2* D = keep_w ? B : s->g_spec_d.as<float2>();
+        float2* D = own_d ? s->g_spec_d.as<float2>() : B;
+        MVD_REQUIRE(!want_psf || !own_d || s->g_spec_d.bytes >= sizeof(float2) * hs, "internal: correlation spectrum");
         const Int3 shift = {(k.x - 1) / 2, (k.y - 1) / 2, (k.z - 1) / 2};
         MVD_CHECK(launch_z_grad(sp, Cs, A, Kh, want_psf ? D : nullptr, keep_w ? 1 : 0, bi == 0, shift, 0,
-                                want_grad_psi ? g.win.z : 0, keep_w && want_psf ? k.z : 0, st));
+                                want_grad_psi ? g.win.z : 0, keep_w && want_psf ? k.z : 0, rsum, st));
         if (want_psf && last) {
           if (!keep_w) MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));
           ConvGeom gk{};
@@ -1883,10 +1916,12 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         continue;
       }
       MVD_CHECK(launch_z(sp, Cs, nullptr, 1.f, 0, q.z, st, -1.f));
+      MVD_CHECK(launch_set_dc(Cs, rsum, st));
       if (want_psf) {
         // dL/dP(j) = sum over blocks of corr(R, W)(c - j) = IFFT(sum conj(R^) W^)(c - j) / (qx qy qz):
         // the products are summed here and inverted once after the last block
-        float2* D = blocks.size() > 1 ? s->g_spec_d.as<float2>() : B;
+        float2* D = own_d ? s->g_spec_d.as<float2>() : B;
+        MVD_REQUIRE(!own_d || s->g_spec_d.bytes >= sizeof(float2) * hs, "internal: correlation spectrum");
         MVD_CHECK(launch_spec_corr_acc(D, Cs, A, bi == 0, hs, st));
         if (last) {
           MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));

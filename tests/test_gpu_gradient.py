@@ -34,8 +34,20 @@ def test_gradient_matches_fp64_oracle():
     ref_nll, ref_gpsi, ref_gpsf = O.nll_grad([v.astype(np.float64) for v in ds.views],
                                              [p.astype(np.float64) for p in start], psi.astype(np.float64))
     assert abs(nll - ref_nll) / abs(ref_nll) < 1e-6
-    for g, r in zip(gpsf, ref_gpsf):
-        assert np.abs(g - r).max() / np.abs(r).max() < 1e-4
+    # dL/dP(j) = sum_x E(psi)(x + c - j) (1 - y / b): scaling the kernel by
+    # 1 + e moves every PSF voxel's gradient by about e sum(y), here ~1e-3 of
+    # the gradient per ulp of e.  The FP32 kernel normalisation is exact only
+    # to its last bit, so the gradient is compared in two parts: its shape
+    # (the constant offset removed) at 1e-4, and the offset within what two
+    # ulps of the normalisation give.
+    for v, (g, r) in enumerate(zip(gpsf, ref_gpsf)):
+        diff = g.astype(np.float64) - r
+        off = float(diff.mean())
+        shape_err = float(np.abs(diff - off).max() / np.abs(r).max())
+        bound = 2 * 2.0 ** -24 * float(ds.views[v].sum(dtype=np.float64))
+        print(f"view {v}: dP shape err {shape_err:.2e}, offset {off:.3f} (2-ulp bound {bound:.3f})", flush=True)
+        assert shape_err < 1e-4
+        assert abs(off) < bound
     assert np.abs(gpsi - ref_gpsi).max() / np.abs(ref_gpsi).max() < 1e-4
     d.close()
 
@@ -95,4 +107,43 @@ def test_blocked_gradient_matches_oracle(block):
     np.testing.assert_allclose(d.grad_psi(), gpsi, rtol=0, atol=1e-6 * np.abs(gpsi).max())
     for a, b in zip(gpsf, gpsf2):
         np.testing.assert_array_equal(a, b)
+    d.close()
+
+
+@pytest.mark.parametrize("size,k", [((260, 20, 30), (29, 5, 5)), ((20, 260, 250), (5, 29, 39))])
+def test_gradient_288_point_windows(size, k):
+    """Windows of 288 points (image + kernel - 1) along x (a 144-point complex
+    transform of the packed row) or along y and z take the 288 = 24 x 12
+    register kernels, including the gradient z pass."""
+    # start PSFs off by up to 20 % in width, as in a refinement
+    ds = S.make_dataset(size, 2, [0, 90], 12, k, 1.2, 2.0, seed=sum(size))
+    start = S.perturbed_psfs(k, 1.2, 2.0, [0, 90], seed=sum(size))
+    inp = MVDeconInput()
+    for v, p in zip(ds.views, start):
+        inp.add(MVDeconFFT(v, None, p, [0]))
+    d = MVDeconvolution(inp, PSFTYPE.EFFICIENT_BAYESIAN, 2, 0.0)
+    print("gradient 288", size, k, d.geometry(0), flush=True)
+    psi = d.get_psi(masked=False)
+    ref_psi, _ = O.deconvolve(ds.views, start, O.EFFICIENT_BAYESIAN, 2, 0.0, norm_threads=8)
+    assert np.abs(psi - ref_psi).max() / np.abs(ref_psi).max() < 1e-4
+    kernels = [v.get_kernel1() for v in d.get_data().get_views()]
+    nll, gpsi, gpsf = d.nll_grad(psfs=kernels)
+    ref_nll, ref_gpsi, ref_gpsf = O.nll_grad([v.astype(np.float64) for v in ds.views],
+                                             [p.astype(np.float64) for p in kernels], psi.astype(np.float64))
+    assert abs(nll - ref_nll) / abs(ref_nll) < 1e-6
+    # dL/dP(j) = sum_x E(psi)(x + c - j) (1 - y / b): scaling the kernel by
+    # 1 + e moves every PSF voxel's gradient by about e sum(y), here ~1e-3 of
+    # the gradient per ulp of e.  The FP32 kernel normalisation is exact only
+    # to its last bit, so the gradient is compared in two parts: its shape
+    # (the constant offset removed) at 1e-4, and the offset within what two
+    # ulps of the normalisation give.
+    for v, (g, r) in enumerate(zip(gpsf, ref_gpsf)):
+        diff = g.astype(np.float64) - r
+        off = float(diff.mean())
+        shape_err = float(np.abs(diff - off).max() / np.abs(r).max())
+        bound = 2 * 2.0 ** -24 * float(ds.views[v].sum(dtype=np.float64))
+        print(f"view {v}: dP shape err {shape_err:.2e}, offset {off:.3f} (2-ulp bound {bound:.3f})", flush=True)
+        assert shape_err < 1e-4
+        assert abs(off) < bound
+    assert np.abs(gpsi - ref_gpsi).max() / np.abs(ref_gpsi).max() < 1e-4
     d.close()

@@ -44,6 +44,9 @@ def test_device_query(jna):
     ((64, 64, 64), (13, 11, 9)), ((128, 128, 128), (31, 19, 19)), ((64, 256, 256), (51, 31, 31)),
     ((64, 64, 512), (9, 9, 41)), ((32, 64, 1024), (5, 7, 9)), ((512, 64, 64), (81, 9, 9)),
     ((256, 128, 128), (3, 3, 3)),
+    # 288- and 144-point lengths (288 = 24 x 12, 144 = 12 x 12) take the
+    # register-resident kernels too; 100 (x) the generic ones
+    ((288, 144, 144), (31, 19, 19)), ((100, 288, 280), (9, 31, 51)),
     # odd sizes above the staging threshold: two 16 MiB chunks, the second ragged
     ((150, 170, 190), (7, 5, 9))])
 def test_jna_inplace_interior_matches_linear_convolution(jna, bshape, kshape):
@@ -160,9 +163,13 @@ def _small_dataset(seed=7, size=(40, 36, 24), nv=3):
     (PSFTYPE.OPTIMIZATION_I, 0.006, (24, 20, 16)),
     (PSFTYPE.INDEPENDENT, 0.0, (30, 30, 20)),
     (PSFTYPE.OPTIMIZATION_II, 0.004, (40, 36, 24)),
+    # one window of 144 x 144 x 144 (image + kernel - 1 = 144 x 136 x 144)
+    (PSFTYPE.EFFICIENT_BAYESIAN, 0.006, "144"),
 ])
 def test_deconvolution_matches_oracle(ptype, lam, block):
-    ds = _small_dataset()
+    ds = _small_dataset(size=(138, 130, 136)) if block == "144" else _small_dataset()
+    if block == "144":
+        block = None
     print("deconv", ptype.name, lam, block, flush=True)
     d = _session(ds.views, ds.psfs, ptype, 6, lam, block=block)
     psi = d.get_psi()
