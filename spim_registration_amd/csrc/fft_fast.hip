@@ -676,6 +676,51 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     buf[l * MP + k] = za;
   }
   __syncthreads();
+  const EpilogueArgs& e = P.e;
+  const int nx = g.eff_size.x;
+  const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
+  const int lsub = RPI > 1 ? threadIdx.x / W : 0;
+  constexpr int NR = L / RPI;
+  // The epilogue runs over the thread's rows in chunks of CR rows (about 8
+  // voxels per thread): a chunk's destination indices and inputs (the view or
+  // the estimate, and the weight) are loaded while the chunk before it is
+  // computed, and the first chunk's loads are issued before the transform.
+  constexpr int CR = EPT >= 8 ? 1 : (8 / EPT < NR ? 8 / EPT : NR);
+  static_assert(NR % CR == 0, "epilogue chunks");
+  constexpr int NCH = NR / CR;
+  struct Chunk {
+    int gi[CR][EPT];  // images hold fewer than 2^31 voxels (checked by the host)
+    float in0[CR][EPT], in1[CR][EPT];
+  };
+  auto load_chunk = [&](int c, Chunk& ch) {
+#pragma unroll
+    for (int rr = 0; rr < CR; ++rr) {
+      const int row = row0 + (c * CR + rr) * RPI + lsub;
+      const bool live_row = row < nrows;
+      const int yo = live_row ? row % ny : 0, zo = live_row ? row / ny : 0;
+      const int base = g.eff_offset.x + g.dst.x * ((g.eff_offset.y + yo) + g.dst.y * (g.eff_offset.z + zo));
+#pragma unroll
+      for (int ee = 0; ee < EPT; ++ee) {
+        const int xo = x0 + ee * NT;
+        const bool live = live_row && xo < nx;
+        const int gi = live ? base + xo : -1;
+        ch.gi[rr][ee] = gi;
+        ch.in0[rr][ee] = 0.f;
+        ch.in1[rr][ee] = 1.f;
+        if (live) {
+          if (EPI == kEpiQuotient || EPI == kEpiResidual) {
+            ch.in0[rr][ee] = gmem(e.img)[gi];
+            if (EPI == kEpiResidual && e.weight_mode == kWeightArray) ch.in1[rr][ee] = gmem(e.weight)[gi];
+          } else if (EPI == kEpiUpdate) {
+            ch.in0[rr][ee] = gmem(e.dst)[gi];
+            ch.in1[rr][ee] = e.weight_mode != kWeightNone ? gmem(e.weight)[gi] : e.wconst;
+          }
+        }
+      }
+    }
+  };
+  Chunk cur;
+  load_chunk(0, cur);
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   auto addr = [line](int i) { return line * MP + i; };
   float2 v[E];
@@ -691,78 +736,53 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
   __syncthreads();
   const float* fbuf = reinterpret_cast<const float*>(buf);
-  const EpilogueArgs& e = P.e;
   const double inv_lambda = rl_inv_lambda(e.lambda);
   double my_sum = 0.0;
   float my_max = -1.f;
-  const int nx = g.eff_size.x;
-  const int x0 = RPI > 1 ? threadIdx.x % W : threadIdx.x;
-  const int lsub = RPI > 1 ? threadIdx.x / W : 0;
-  constexpr int NR = L / RPI;
-  // phase 1: destination indices and the inputs of the epilogue, all rows
-  int gidx[NR][EPT];  // images hold fewer than 2^31 voxels (checked by the host)
-  float in0[NR][EPT], in1[NR][EPT];
+  auto run_chunk = [&](int c, const Chunk& ch) {
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int row = row0 + r * RPI + lsub;
-    const bool live_row = row < nrows;
-    const int yo = live_row ? row % ny : 0, zo = live_row ? row / ny : 0;
-    const int base = g.eff_offset.x + g.dst.x * ((g.eff_offset.y + yo) + g.dst.y * (g.eff_offset.z + zo));
+    for (int rr = 0; rr < CR; ++rr) {
+      const int l = (c * CR + rr) * RPI + lsub;
 #pragma unroll
-    for (int ee = 0; ee < EPT; ++ee) {
-      const int xo = x0 + ee * NT;
-      const bool live = live_row && xo < nx;
-      const int gi = live ? base + xo : -1;
-      gidx[r][ee] = gi;
-      in0[r][ee] = 0.f;
-      in1[r][ee] = 1.f;
-      if (live) {
-        if (EPI == kEpiQuotient || EPI == kEpiResidual) {
-          in0[r][ee] = gmem(e.img)[gi];
-          if (EPI == kEpiResidual && e.weight_mode == kWeightArray) in1[r][ee] = gmem(e.weight)[gi];
-        } else if (EPI == kEpiUpdate) {
-          in0[r][ee] = gmem(e.dst)[gi];
-          in1[r][ee] = e.weight_mode != kWeightNone ? gmem(e.weight)[gi] : e.wconst;
-        }
-      }
-    }
-  }
-  // phase 2: compute and store
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int l = r * RPI + lsub;
-#pragma unroll
-    for (int ee = 0; ee < EPT; ++ee) {
-      const int gi = gidx[r][ee];
-      if (gi < 0) continue;
-      const int xo = x0 + ee * NT;
-      const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
-      if (EPI == kEpiStore) {
-        gmem(e.dst)[gi] = val;
-      } else if (EPI == kEpiQuotient) {
-        gmem(e.dst)[gi] = rl_quotient(val, in0[r][ee]);
-      } else if (EPI == kEpiResidual) {
-        const double w = (double)in1[r][ee], b = (double)val, y = (double)in0[r][ee];
-        if (b > 0) {
-          my_sum += w * (b - y * log(b));
-          gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
+      for (int ee = 0; ee < EPT; ++ee) {
+        const int gi = ch.gi[rr][ee];
+        if (gi < 0) continue;
+        const int xo = x0 + ee * NT;
+        const float val = fbuf[l * 2 * MP + g.eff_local.x + xo];
+        if (EPI == kEpiStore) {
+          gmem(e.dst)[gi] = val;
+        } else if (EPI == kEpiQuotient) {
+          gmem(e.dst)[gi] = rl_quotient(val, ch.in0[rr][ee]);
+        } else if (EPI == kEpiResidual) {
+          const double w = (double)ch.in1[rr][ee], b = (double)val, y = (double)ch.in0[rr][ee];
+          if (b > 0) {
+            my_sum += w * (b - y * log(b));
+            gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
+          } else {
+            gmem(e.dst)[gi] = 0.f;
+          }
         } else {
-          gmem(e.dst)[gi] = 0.f;
+          float w = ch.in1[rr][ee];
+          if (e.weight_mode == kWeightVirtual) {
+            const double rw = (double)ch.in1[rr][ee] / (double)gmem(e.wsum)[gi];
+            w = (float)java_min(1.0, rw * e.osem);
+          }
+          const float last = ch.in0[rr][ee];
+          const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
+          gmem(e.dst)[gi] = next;
+          const float change = fabsf(next - last);
+          my_sum += (double)change;
+          my_max = java_maxf(my_max, change);
         }
-      } else {
-        float w = in1[r][ee];
-        if (e.weight_mode == kWeightVirtual) {
-          const double rr = (double)in1[r][ee] / (double)gmem(e.wsum)[gi];
-          w = (float)java_min(1.0, rr * e.osem);
-        }
-        const float last = in0[r][ee];
-        const float next = rl_next_value(last, val, w, e.lambda, inv_lambda);
-        gmem(e.dst)[gi] = next;
-        const float change = fabsf(next - last);
-        my_sum += (double)change;
-        my_max = java_maxf(my_max, change);
       }
     }
+  };
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    Chunk nxt;
+    if (c + 1 < NCH) load_chunk(c + 1, nxt);
+    run_chunk(c, cur);
+    if (c + 1 < NCH) cur = nxt;
   }
   if (EPI == kEpiUpdate || EPI == kEpiResidual) {
     for (int o = 32; o > 0; o >>= 1) {

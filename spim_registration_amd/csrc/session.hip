@@ -1760,6 +1760,14 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
   }
   MVD_CHECK(s->g_nll.ensure(sizeof(double) * 2 * s->nviews));
   float* resid = s->d_ratio.as<float>();
+  // the PSF gradients of all views in one device buffer, copied to the host
+  // after the last view (a copy to pageable memory waits for the stream)
+  std::vector<int64_t> kofs(s->nviews + 1, 0);
+  for (int v = 0; v < s->nviews; ++v) {
+    const Int3 kd = s->views[v].kd;
+    kofs[v + 1] = kofs[v] + (grad_psf && grad_psf[v] ? (int64_t)kd.x * kd.y * kd.z : 0);
+  }
+  if (kofs[s->nviews] > 0) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kofs[s->nviews]));
   for (int v = 0; v < s->nviews; ++v) {
     auto& view = s->views[v];
     const Int3 k = view.kd;
@@ -1802,7 +1810,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
     MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
     MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nparts));
-    constexpr int kRsumParts = 256;
+    constexpr int kRsumParts = 1024;
     MVD_CHECK(s->g_rsum.ensure(sizeof(double) * (kRsumParts + 1)));
     double* rsum = s->g_rsum.as<double>() + kRsumParts;
     // the kernel spectrum: the prepared view's own when the PSF is its kernel1
@@ -1845,8 +1853,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
                                   s->g_nll.as<double>() + 2 * v, st));
     const bool want_psf = grad_psf && grad_psf[v];
     if (!want_grad_psi && !want_psf) continue;
-    const int64_t kn = (int64_t)k.x * k.y * k.z;
-    if (want_psf) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kn));
+    float* kgrad = s->g_kgrad.as<float>() + kofs[v];
     const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
     // pass 2, every block, with the window spectrum recomputed instead of kept
     // (checkpointing: one block's spectra in memory at a time)
@@ -1904,7 +1911,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
           es.mode = kEpiStore;
           es.dst = s->g_box.as<float>();
           MVD_CHECK(launch_yx_inverse_passes(sp, gk, D, es, st, nullptr));
-          MVD_CHECK(launch_extract_psf_grad_shifted(s->g_box.as<float>(), k, scale, s->g_kgrad.as<float>(), st));
+          MVD_CHECK(launch_extract_psf_grad_shifted(s->g_box.as<float>(), k, scale, kgrad, st));
         }
         if (want_grad_psi) {
           EpilogueArgs es{};
@@ -1935,7 +1942,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
           es.mode = kEpiStore;
           es.dst = s->g_box.as<float>();
           MVD_CHECK(launch_yx_inverse_passes(sp, gb, D, es, st, nullptr));
-          MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, s->g_kgrad.as<float>(), 0, st));
+          MVD_CHECK(launch_extract_psf_grad(s->g_box.as<float>(), q, k, scale, kgrad, 0, st));
         }
       }
       if (want_grad_psi) {
@@ -1949,9 +1956,11 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
       }
     }
-    if (want_psf)
-      MVD_CHECK(hipMemcpyAsync(grad_psf[v], s->g_kgrad.p, sizeof(float) * kn, hipMemcpyDeviceToHost, st));
   }
+  for (int v = 0; v < s->nviews; ++v)
+    if (kofs[v + 1] > kofs[v])
+      MVD_CHECK(hipMemcpyAsync(grad_psf[v], s->g_kgrad.as<float>() + kofs[v], sizeof(float) * (kofs[v + 1] - kofs[v]),
+                               hipMemcpyDeviceToHost, st));
   std::vector<double> sums(2 * s->nviews);
   MVD_CHECK(hipMemcpyAsync(sums.data(), s->g_nll.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st));
   if (want_grad_psi) {
