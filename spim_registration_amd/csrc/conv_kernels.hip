@@ -153,6 +153,7 @@ struct ZParams {
   float2* spec;
   const float2* kspec;
   float2* aux;  // with kspec: the forward spectrum before the product, all rows (nullable)
+  const float2* pre;  // with kspec: the lines' spectrum already transformed along z (nullable)
   float scale, dir;
   int hx, pitch, qy, qz, zlo, zhi, lines;
   FftPlan plan;
@@ -169,13 +170,14 @@ __device__ __forceinline__ void z_body(const ZParams& P, int bx, int by) {
   const int y = by;
   const int64_t plane = (int64_t)P.qy * P.pitch;
   const int64_t col = (int64_t)y * P.pitch + c0;
+  const float2* in = P.kspec && P.pre ? P.pre : P.spec;
   for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
     const int z = idx / B, b = idx - z * B;
     buf[z * lp + b] = make_float2(0.f, 0.f);
-    if (c0 + b < P.hx) buf[z * lp + b] = gmem(P.spec)[z * plane + col + b];
+    if (c0 + b < P.hx) buf[z * lp + b] = gmem(in)[z * plane + col + b];
   }
   __syncthreads();
-  lds_fft(buf, lp, B, P.plan, tw, 1, P.kspec ? -1.f : P.dir);
+  if (!(P.kspec && P.pre)) lds_fft(buf, lp, B, P.plan, tw, 1, P.kspec ? -1.f : P.dir);
   if (P.kspec) {
     for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
       const int z = idx / B, b = idx - z * B;
@@ -470,10 +472,11 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
 }
 
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream, float dir, float2* aux) {
-  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir, aux);
+                    hipStream_t stream, float dir, float2* aux, const float2* pre) {
+  if (fast_size(sp.q.z)) return launch_z_fast(sp, spec, kspec, scale, zlo, zhi, stream, dir, aux, pre);
   ZParams P;
   P.aux = aux;
+  P.pre = pre;
   P.dir = dir;
   P.spec = spec;
   P.kspec = kspec;
@@ -632,6 +635,7 @@ int build_generic_params(const PassDesc& d, void* out, int* gx, int* gy, size_t*
     case kPassZConv: {
       ZParams P{};
       P.aux = nullptr;
+      P.pre = nullptr;
       P.spec = d.spec;
       P.kspec = d.kspec;
       P.scale = d.scale;

@@ -91,6 +91,9 @@ struct ZFastParams {
   // with kspec (single launches only): the forward spectrum before the
   // product, all rows (nullable)
   float2* aux = nullptr;
+  // with kspec (single launches only): the lines' spectrum already
+  // transformed along z, in place of spec (nullable)
+  const float2* pre = nullptr;
 };
 
 template <int N>
@@ -107,17 +110,30 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
   // spectra hold fewer than 2^31 elements (lengths up to 1024): 32-bit offsets
   const int plane = P.qy * P.pitch;
   const int off = by * P.pitch + col;
-  float2 v[E];
-#pragma unroll
-  for (int b = 0; b < E / S::R0; ++b)
-#pragma unroll
-    for (int t = 0; t < S::R0; ++t) {
-      const int z = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = make_float2(0.f, 0.f);
-      if (valid) v[b * S::R0 + t] = maybe_stream_load<kStream>(&gmem(P.spec)[z * plane + off]);
-    }
-  auto addr = [line](int i) { return i * LP + line; };
+  const bool pre = P.kspec && P.pre;
   constexpr int RL = last_radix<N>();
+  float2 v[E];
+  if (pre) {
+    // already transformed: the lines in the order the forward transform leaves them
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int z = elem_index<N, E, RL>(jt, b, t);
+        v[b * RL + t] = make_float2(0.f, 0.f);
+        if (valid) v[b * RL + t] = gmem(P.pre)[z * plane + off];
+      }
+  } else {
+#pragma unroll
+    for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+      for (int t = 0; t < S::R0; ++t) {
+        const int z = elem_index<N, E, S::R0>(jt, b, t);
+        v[b * S::R0 + t] = make_float2(0.f, 0.f);
+        if (valid) v[b * S::R0 + t] = maybe_stream_load<kStream>(&gmem(P.spec)[z * plane + off]);
+      }
+  }
+  auto addr = [line](int i) { return i * LP + line; };
   // the kernel spectrum is loaded with the line, in the order the forward
   // transform leaves its outputs, so its latency hides behind the transform
   float2 kv[E];
@@ -131,7 +147,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
         if (valid) kv[b * RL + t] = gmem(P.kspec)[z * plane + off];
       }
   }
-  fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
+  if (!pre) fft_forward_order<N>(v, lds, jt, tw, 1, P.kspec ? -1.f : P.dir, addr);
   if (P.kspec) {
     if (P.aux && valid) {
 #pragma unroll
@@ -1043,8 +1059,8 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
 }
 
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream, float dir, float2* aux) {
-  ZFastParams P{spec, kspec, sp.tw_z, scale, dir, sp.hx, sp.pitch, sp.q.y, zlo, zhi, nullptr, nullptr, 0, aux};
+                         hipStream_t stream, float dir, float2* aux, const float2* pre) {
+  ZFastParams P{spec, kspec, sp.tw_z, scale, dir, sp.hx, sp.pitch, sp.q.y, zlo, zhi, nullptr, nullptr, 0, aux, pre};
   (void)hipGetLastError();
 #define MVD_Z(N)                                                                                \
   case N: {                                                                                     \

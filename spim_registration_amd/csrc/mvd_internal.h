@@ -214,9 +214,11 @@ hipError_t launch_y(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo
                     hipStream_t stream);
 // kspec == nullptr: one z transform in direction `dir`, then multiply by `scale`
 // (kernel spectra, gradient passes).  aux (with kspec): also store the forward
-// spectrum before the product, all rows.
+// spectrum before the product, all rows.  pre (with kspec): the lines are
+// read from pre, already transformed along z (no forward transform); the
+// output rows still go to spec.
 hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                    hipStream_t stream, float dir = -1.f, float2* aux = nullptr);
+                    hipStream_t stream, float dir = -1.f, float2* aux = nullptr, const float2* pre = nullptr);
 // Gradient z pass (z_grad_supported): forward R^ of the
 // y-transformed residual in spec; adjoint R^ conj(P^) inverted into rows
 // [zlo, zhi) of spec; correlation conj(R^) W^ with the lag shift `shift`
@@ -276,7 +278,7 @@ int x_fast_rows_per_wg(int M);
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                          hipStream_t stream);
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
-                         hipStream_t stream, float dir, float2* aux = nullptr);
+                         hipStream_t stream, float dir, float2* aux = nullptr, const float2* pre = nullptr);
 hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
                               int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
                               hipStream_t stream);

@@ -1768,6 +1768,15 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     kofs[v + 1] = kofs[v] + (grad_psf && grad_psf[v] ? (int64_t)kd.x * kd.y * kd.z : 0);
   }
   if (kofs[s->nviews] > 0) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kofs[s->nviews]));
+  // W^ = FFT(E_mirror(psi) on a block window), kept in g_spec_a across the
+  // views that share the window (one block per view, same FFT shape): views of
+  // one image with kernels of one size convolve the same window
+  const ShapePlan* w_plan = nullptr;
+  ConvGeom w_geom{};
+  auto same_window = [](const ConvGeom& a, const ConvGeom& b) {
+    auto eq = [](Int3 u, Int3 w) { return u.x == w.x && u.y == w.y && u.z == w.z; };
+    return eq(a.src, b.src) && eq(a.off, b.off) && eq(a.win, b.win) && eq(a.q, b.q);
+  };
   for (int v = 0; v < s->nviews; ++v) {
     auto& view = s->views[v];
     const Int3 k = view.kd;
@@ -1838,12 +1847,19 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       e.part_sum = s->g_part_sum.as<double>() + off;
       e.part_max = s->g_part_max.as<float>() + off;
       if (keep_w) {
-        // W^ stored by the z pass before the product (into A), the convolution in B
+        // W^ in A (transformed once for the views sharing the window), the
+        // convolution W^ P^ transformed back along z into B
+        if (!(w_plan == spp && same_window(w_geom, g))) {
+          MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, A, st));
+          MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
+          w_plan = spp;
+          w_geom = g;
+        }
         const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
-        MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, B, st));
-        MVD_CHECK(launch_z(sp, B, Kh, 1.f, zlo, zhi, st, -1.f, A));
+        MVD_CHECK(launch_z(sp, B, Kh, 1.f, zlo, zhi, st, -1.f, nullptr, A));
         MVD_CHECK(launch_yx_inverse_passes(sp, g, B, e, st, nullptr));
       } else {
+        w_plan = nullptr;  // A is overwritten
         int rc = run_conv(s, sp, g, psrc, kGatherMirror, Kh, e, A);
         if (rc) return rc;
       }
@@ -1863,6 +1879,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         // W^ = FFT(E_mirror(psi) on the block window)
         MVD_CHECK(launch_xy_forward_passes(sp, g, psrc, kGatherMirror, A, st));
         MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
+        w_plan = nullptr;
       }
       // R^ = FFT(residual of this block's effective region, zero elsewhere in the
       // window); the residual image itself when that region is the whole image
