@@ -19,6 +19,8 @@ struct YFastParams {
   const float2* tw;
   int hx, pitch, zlo, ylo, yhi;
   float dir;
+  // input rows outside [iylo, iyhi) of planes outside [izlo, izhi) read as zero
+  int iylo = 0, iyhi = 1 << 30, izlo = 0, izhi = 1 << 30;
 };
 
 template <int N>
@@ -36,13 +38,15 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
   const bool valid = col < P.hx, store_col = col < P.pitch;
   const int z = P.zlo + by;
   const auto base = gmem(P.spec) + (int64_t)z * N * P.pitch + col;
+  const bool plane_in = z >= P.izlo && z < P.izhi;
   float2 v[E];
 #pragma unroll
   for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
     for (int t = 0; t < S::R0; ++t) {
       const int y = elem_index<N, E, S::R0>(jt, b, t);
-      v[b * S::R0 + t] = valid ? maybe_stream_load<kStream>(&base[y * P.pitch]) : make_float2(0.f, 0.f);
+      const bool in = valid && plane_in && y >= P.iylo && y < P.iyhi;
+      v[b * S::R0 + t] = in ? maybe_stream_load<kStream>(&base[y * P.pitch]) : make_float2(0.f, 0.f);
     }
   auto addr = [line](int i) { return i * LP + line; };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
@@ -467,6 +471,65 @@ struct XFwdFastParams {
   int pitch;
 };
 
+// Real-to-complex transform of the workgroup's L rows of 2M real samples held
+// in LDS (row l at buf + l (M + 1), samples x-fastest, as M complex pairs) and
+// the store of X[0, M] with the row padding [M + 1, M + 16) as zeros, so that
+// the row's last 128-byte line is written by one store (a partly written line
+// reaches HBM as a read-modify-write).  row_out(l): the row's output in global
+// memory, or null for rows that are not stored.  The caller has synchronised
+// after filling the rows.
+template <int M, class RowOut>
+__device__ __forceinline__ void r2c_rows_store(float2* buf, const float2* tw, RowOut row_out) {
+  using S = Spec<M>;
+  constexpr int E = S::E, T = M / E, NT = S::NT, L = NT / T, MP = M + 1;
+  const int line = threadIdx.x % L, jt = threadIdx.x / L;
+  auto addr = [line](int i) { return line * MP + i; };
+  float2 v[E];
+#pragma unroll
+  for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+    for (int t = 0; t < S::R0; ++t) v[b * S::R0 + t] = buf[addr(elem_index<M, E, S::R0>(jt, b, t))];
+  fft_forward_order<M>(v, buf, jt, tw, 2, -1.f, addr);
+  constexpr int RL = last_radix<M>();
+#pragma unroll
+  for (int b = 0; b < E / RL; ++b)
+#pragma unroll
+    for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
+  __syncthreads();
+  // unpack Z = FFT(even + i odd) into X[k] for the pair (k, M - k); X[M] (the
+  // pair of k = 0) goes out with the padding
+  constexpr int K2 = M / 2 + 1;
+  for (int idx = threadIdx.x; idx < L * K2; idx += NT) {
+    const int l = idx / K2, k = idx % K2;
+    const auto out = row_out(l);
+    if (!out) continue;
+    const float2 a = buf[l * MP + k], b = buf[l * MP + (M - k) % M];
+    {
+      const float2 bc = cconj(b);
+      const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
+      stream_store(&out[k], cadd(E2, cmul(tw[k], make_float2(D.y, -D.x))));
+    }
+    if (M - k != k && k != 0) {
+      const float2 ac = cconj(a);
+      const float2 E2 = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
+      stream_store(&out[M - k], cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x))));
+    }
+  }
+  for (int idx = threadIdx.x; idx < L * 16; idx += NT) {
+    const int l = idx >> 4, c = idx & 15;
+    const auto out = row_out(l);
+    if (!out) continue;
+    float2 x = make_float2(0.f, 0.f);
+    if (c == 0) {
+      const float2 a = buf[l * MP];
+      const float2 ac = cconj(a);
+      const float2 E2 = cscale(cadd(a, ac), 0.5f), D = cscale(csub(a, ac), 0.5f);
+      x = cadd(E2, cmul(tw[M], make_float2(D.y, -D.x)));
+    }
+    stream_store(&out[M + c], x);
+  }
+}
+
 // One row of the window: rows are (y, z) of the FFT box, x-fastest.  Loops
 // run over whole rows so that the (y, z) bookkeeping is done once per row.
 template <int M, int MODE>
@@ -549,55 +612,10 @@ __device__ __forceinline__ void x_forward_fast_body(const XFwdFastParams& P, int
     for (int e = 0; e < EPT; ++e)
       if (x_in_row(e)) fbuf[(r * RPI + lsub) * 2 * MP + x0 + e * NT] = vals[r][e];
   __syncthreads();
-  const int line = threadIdx.x % L, jt = threadIdx.x / L;
-  auto addr = [line](int i) { return line * MP + i; };
-  float2 v[E];
-#pragma unroll
-  for (int b = 0; b < E / S::R0; ++b)
-#pragma unroll
-    for (int t = 0; t < S::R0; ++t) v[b * S::R0 + t] = buf[addr(elem_index<M, E, S::R0>(jt, b, t))];
-  fft_forward_order<M>(v, buf, jt, tw, 2, -1.f, addr);
-  constexpr int RL = last_radix<M>();
-#pragma unroll
-  for (int b = 0; b < E / RL; ++b)
-#pragma unroll
-    for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
-  __syncthreads();
-  // unpack Z = FFT(even + i odd) into X[k] for the pair (k, M - k); X[M] (the
-  // pair of k = 0) goes out below with the row padding [M + 1, M + 16), zeros,
-  // so that the row's last 128-byte line is written by one store (a partly
-  // written line reaches HBM as a read-modify-write)
-  constexpr int K2 = M / 2 + 1;
-  for (int idx = threadIdx.x; idx < L * K2; idx += NT) {
-    const int l = idx / K2, k = idx % K2;
+  r2c_rows_store<M>(buf, tw, [&](int l) {
     const int row = row0 + l;
-    if (row >= nrows) continue;
-    const float2 a = buf[l * MP + k], b = buf[l * MP + (M - k) % M];
-    const auto out = gmem(P.spec) + (int64_t)row * P.pitch;
-    {
-      const float2 bc = cconj(b);
-      const float2 E2 = cscale(cadd(a, bc), 0.5f), D = cscale(csub(a, bc), 0.5f);
-      stream_store(&out[k], cadd(E2, cmul(tw[k], make_float2(D.y, -D.x))));
-    }
-    if (M - k != k && k != 0) {
-      const float2 ac = cconj(a);
-      const float2 E2 = cscale(cadd(b, ac), 0.5f), D = cscale(csub(b, ac), 0.5f);
-      stream_store(&out[M - k], cadd(E2, cmul(tw[M - k], make_float2(D.y, -D.x))));
-    }
-  }
-  for (int idx = threadIdx.x; idx < L * 16; idx += NT) {
-    const int l = idx >> 4, c = idx & 15;
-    const int row = row0 + l;
-    if (row >= nrows) continue;
-    float2 v = make_float2(0.f, 0.f);
-    if (c == 0) {
-      const float2 a = buf[l * MP];
-      const float2 ac = cconj(a);
-      const float2 E2 = cscale(cadd(a, ac), 0.5f), D = cscale(csub(a, ac), 0.5f);
-      v = cadd(E2, cmul(tw[M], make_float2(D.y, -D.x)));
-    }
-    stream_store(&gmem(P.spec)[(int64_t)row * P.pitch + M + c], v);
-  }
+    return row < nrows ? gmem(P.spec) + (int64_t)row * P.pitch : nullptr;
+  });
 }
 
 template <int M, int MODE>
@@ -639,6 +657,7 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
   __shared__ float2 buf[L * MP];
   __shared__ double red_sum[NT / 64];
   __shared__ float red_max[NT / 64];
+  __shared__ double red_rsum[NT / 64];
   const ConvGeom& g = P.g;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = gmem(P.tw)[i];
   const int ny = g.eff_size.y;
@@ -751,10 +770,12 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
 #pragma unroll
     for (int t = 0; t < RL; ++t) buf[addr(elem_index<M, E, RL>(jt, b, t))] = v[b * RL + t];
   __syncthreads();
-  const float* fbuf = reinterpret_cast<const float*>(buf);
+  float* fbuf = reinterpret_cast<float*>(buf);
   const double inv_lambda = rl_inv_lambda(e.lambda);
-  double my_sum = 0.0;
+  double my_sum = 0.0, my_rsum = 0.0;
   float my_max = -1.f;
+  // residual rows transformed here (rspec): r goes back into the row in LDS
+  const bool r_rows = EPI == kEpiResidual && e.rspec != nullptr;
   auto run_chunk = [&](int c, const Chunk& ch) {
 #pragma unroll
     for (int rr = 0; rr < CR; ++rr) {
@@ -771,11 +792,16 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
           gmem(e.dst)[gi] = rl_quotient(val, ch.in0[rr][ee]);
         } else if (EPI == kEpiResidual) {
           const double w = (double)ch.in1[rr][ee], b = (double)val, y = (double)ch.in0[rr][ee];
+          float rv = 0.f;
           if (b > 0) {
             my_sum += w * (b - y * log(b));
-            gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
+            rv = (float)(w * (1.0 - y / b));
+          }
+          if (r_rows) {
+            fbuf[l * 2 * MP + g.eff_local.x + xo] = rv;  // the value this thread read
+            my_rsum += (double)rv;
           } else {
-            gmem(e.dst)[gi] = 0.f;
+            gmem(e.dst)[gi] = rv;
           }
         } else {
           float w = ch.in1[rr][ee];
@@ -804,23 +830,42 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
     for (int o = 32; o > 0; o >>= 1) {
       my_sum += __shfl_xor(my_sum, o);
       my_max = java_maxf(my_max, __shfl_xor(my_max, o));
+      if (EPI == kEpiResidual) my_rsum += __shfl_xor(my_rsum, o);
     }
     const int wid = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0) {
       red_sum[wid] = my_sum;
       red_max[wid] = my_max;
+      if (EPI == kEpiResidual) red_rsum[wid] = my_rsum;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
+      double s = 0.0, rs = 0.0;
       float m = -1.f;
       for (int i = 0; i < NT / 64; ++i) {
         s += red_sum[i];
         m = java_maxf(m, red_max[i]);
+        if (EPI == kEpiResidual) rs += red_rsum[i];
       }
       gmem(e.part_sum)[bx] = s;
       gmem(e.part_max)[bx] = m;
+      if (r_rows) gmem(e.part_rsum)[bx] = rs;
     }
+  }
+  if (r_rows) {
+    // window row = the effective row placed at eff_local, zeros elsewhere;
+    // then the x transform of the rows into the residual's window spectrum
+    for (int idx = threadIdx.x; idx < L * W; idx += NT) {
+      const int l = idx / W, x = idx % W;
+      if (x < g.eff_local.x || x >= g.eff_local.x + nx) fbuf[l * 2 * MP + x] = 0.f;
+    }
+    __syncthreads();
+    r2c_rows_store<M>(buf, tw, [&](int l) {
+      const int row = row0 + l;
+      return row < nrows ? gmem(e.rspec) + ((int64_t)(g.eff_local.z + row / ny) * g.q.y + g.eff_local.y + row % ny) *
+                                               e.rpitch
+                         : nullptr;
+    });
   }
 }
 
@@ -1034,9 +1079,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
   return hipGetLastError();
 }
 
-hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
-                         hipStream_t stream) {
-  YFastParams P{spec, sp.tw_y, sp.hx, sp.pitch, zlo, ylo, yhi, dir};
+hipError_t launch_y_fast_window(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                                Int3 in_lo, Int3 in_hi, hipStream_t stream) {
+  YFastParams P{spec, sp.tw_y, sp.hx, sp.pitch, zlo, ylo, yhi, dir, in_lo.y, in_hi.y, in_lo.z, in_hi.z};
   const int n = sp.q.y;
   (void)hipGetLastError();
 #define MVD_Y(N)                                                                                \
@@ -1056,6 +1101,11 @@ hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, in
   }
 #undef MVD_Y
   return hipGetLastError();
+}
+
+hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                         hipStream_t stream) {
+  return launch_y_fast_window(sp, spec, zlo, zhi, ylo, yhi, dir, {0, 0, 0}, {1 << 30, 1 << 30, 1 << 30}, stream);
 }
 
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,

@@ -213,11 +213,16 @@ __global__ __launch_bounds__(kGThreads) void k_reduce_sum(const double* part, in
   if (threadIdx.x == 0) *out = rs[0];
 }
 
+hipError_t launch_reduce_sum(const double* part, int n, double* out, hipStream_t stream) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kGThreads), 0, stream, part, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_sum_to(const float* a, int64_t n, double* part, int nparts, double* out, hipStream_t stream) {
   hipError_t e = launch_sum(a, n, part, nparts, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kGThreads), 0, stream, part, nparts, out);
-  return hipGetLastError();
+  return launch_reduce_sum(part, nparts, out, stream);
 }
 
 // spec[0] = (float) *v: the zero-frequency value of a real line set from its

@@ -205,6 +205,13 @@ struct EpilogueArgs {
   double* part_sum;      // per-workgroup statistics of the update
   float* part_max;
   float wconst = 1.f;    // the weight of every voxel under kWeightNone
+  // kEpiResidual, register x inverse only: instead of storing the residual,
+  // transform its rows (the block's effective box placed in its window, zeros
+  // elsewhere) along x into rspec (window rows, pitch rpitch) and write each
+  // workgroup's residual sum (double) to part_rsum
+  float2* rspec = nullptr;
+  int rpitch = 0;
+  double* part_rsum = nullptr;
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -277,6 +284,10 @@ bool fast_size(int n);
 int x_fast_rows_per_wg(int M);
 hipError_t launch_y_fast(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
                          hipStream_t stream);
+// y pass whose input holds data only in rows [in_lo.y, in_hi.y) of planes
+// [in_lo.z, in_hi.z); every other input row is read as zero (register kernels).
+hipError_t launch_y_fast_window(const ShapePlan& sp, float2* spec, int zlo, int zhi, int ylo, int yhi, float dir,
+                                Int3 in_lo, Int3 in_hi, hipStream_t stream);
 hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec, float scale, int zlo, int zhi,
                          hipStream_t stream, float dir, float2* aux = nullptr, const float2* pre = nullptr);
 hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
@@ -373,6 +384,8 @@ hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 s
 hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream);
 // acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
 hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
+// *out = sum of the partials part[0, n) (one workgroup, a fixed order).
+hipError_t launch_reduce_sum(const double* part, int n, double* out, hipStream_t stream);
 // *out = sum of a[0, n) in double, through nparts partials in part.
 hipError_t launch_sum_to(const float* a, int64_t n, double* part, int nparts, double* out, hipStream_t stream);
 // spec[0] = (float) *v (the zero-frequency value of a spectrum).

@@ -632,9 +632,10 @@ struct mvd_session {
   DeviceBuf d_views, d_weights, d_wsum, d_psi, d_ratio, d_spec, d_part_sum, d_part_max, d_stats, d_tmp;
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
-  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max, g_kgrad;
+  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max;
+  DeviceBuf g_out;  // per call: the views' NLL sums (double) and PSF gradients (float), copied out at once
   DeviceBuf g_rsum;  // partials and the double sum of one block's residual
-  DeviceBuf g_crop, g_nll, g_gpsi;
+  DeviceBuf g_crop, g_gpsi;
   bool g_gpsi_ready = false;
   PlanCache plans;
   Timer timer;
@@ -1758,16 +1759,19 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_acc.ensure(sizeof(double) * s->n));
     MVD_CHECK(launch_zero_double(s->g_acc.as<double>(), s->n, st));
   }
-  MVD_CHECK(s->g_nll.ensure(sizeof(double) * 2 * s->nviews));
   float* resid = s->d_ratio.as<float>();
-  // the PSF gradients of all views in one device buffer, copied to the host
-  // after the last view (a copy to pageable memory waits for the stream)
+  // the views' NLL sums and PSF gradients in one device buffer, copied to the
+  // host in one transfer after the last view (a copy to pageable memory waits
+  // for the stream)
   std::vector<int64_t> kofs(s->nviews + 1, 0);
   for (int v = 0; v < s->nviews; ++v) {
     const Int3 kd = s->views[v].kd;
     kofs[v + 1] = kofs[v] + (grad_psf && grad_psf[v] ? (int64_t)kd.x * kd.y * kd.z : 0);
   }
-  if (kofs[s->nviews] > 0) MVD_CHECK(s->g_kgrad.ensure(sizeof(float) * kofs[s->nviews]));
+  const size_t out_bytes = sizeof(double) * 2 * s->nviews + sizeof(float) * kofs[s->nviews];
+  MVD_CHECK(s->g_out.ensure(out_bytes));
+  double* nll_sums = s->g_out.as<double>();
+  float* kgrads = reinterpret_cast<float*>(nll_sums + 2 * s->nviews);
   // W^ = FFT(E_mirror(psi) on a block window), kept in g_spec_a across the
   // views that share the window (one block per view, same FFT shape): views of
   // one image with kernels of one size convolve the same window
@@ -1811,6 +1815,10 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     // launch; keep_w: one block, whose window spectrum W^ is kept from pass 1
     const bool z_grad = g_grad_fused && z_grad_supported(sp);
     const bool keep_w = z_grad && g_grad_keep_w && blocks.size() == 1;
+    // fuse_r (one block, register x and y passes): the pass-1 x inverse turns
+    // the residual rows into their x transform in the window spectrum Cs and
+    // sums them; the residual image is not written
+    const bool fuse_r = keep_w && fast_size(q.x / 2) && fast_size(q.y) && !xy_fused(sp);
     // the summed correlation spectrum D: its own buffer unless one block's
     // correlation goes through B (keep_w, or the separate launches with one block)
     const bool own_d = z_grad ? !keep_w : blocks.size() > 1;
@@ -1819,9 +1827,11 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(s->g_crop.ensure(sizeof(float) * crop));
     MVD_CHECK(s->g_part_sum.ensure(sizeof(double) * nparts));
     MVD_CHECK(s->g_part_max.ensure(sizeof(float) * nparts));
+    // residual sums: partials (of k_sum, or of the fused x inverse) and the sum
     constexpr int kRsumParts = 1024;
-    MVD_CHECK(s->g_rsum.ensure(sizeof(double) * (kRsumParts + 1)));
-    double* rsum = s->g_rsum.as<double>() + kRsumParts;
+    const int rparts = std::max(kRsumParts, nparts);
+    MVD_CHECK(s->g_rsum.ensure(sizeof(double) * (rparts + 1)));
+    double* rsum = s->g_rsum.as<double>() + rparts;
     // the kernel spectrum: the prepared view's own when the PSF is its kernel1
     const float2* Kh = nullptr;
     if (s->prepared && view.plan == spp && view.k1hat.p && pk == view.k1) {
@@ -1857,7 +1867,17 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         }
         const int zlo = g.eff_local.z, zhi = g.eff_local.z + g.eff_size.z;
         MVD_CHECK(launch_z(sp, B, Kh, 1.f, zlo, zhi, st, -1.f, nullptr, A));
-        MVD_CHECK(launch_yx_inverse_passes(sp, g, B, e, st, nullptr));
+        if (fuse_r) {
+          e.rspec = s->g_spec_c.as<float2>();
+          e.rpitch = sp.pitch;
+          e.part_rsum = s->g_rsum.as<double>();
+        }
+        int nwg = 0;
+        MVD_CHECK(launch_yx_inverse_passes(sp, g, B, e, st, &nwg));
+        if (fuse_r) {
+          MVD_REQUIRE(nwg <= rparts, "internal: residual partials");
+          MVD_CHECK(launch_reduce_sum(s->g_rsum.as<double>(), nwg, rsum, st));
+        }
       } else {
         w_plan = nullptr;  // A is overwritten
         int rc = run_conv(s, sp, g, psrc, kGatherMirror, Kh, e, A);
@@ -1866,10 +1886,10 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       off += conv_inverse_workgroups(sp, g);
     }
     MVD_CHECK(launch_reduce_stats(s->g_part_sum.as<double>(), s->g_part_max.as<float>(), off,
-                                  s->g_nll.as<double>() + 2 * v, st));
+                                  nll_sums + 2 * v, st));
     const bool want_psf = grad_psf && grad_psf[v];
     if (!want_grad_psi && !want_psf) continue;
-    float* kgrad = s->g_kgrad.as<float>() + kofs[v];
+    float* kgrad = kgrads + kofs[v];
     const float scale = (float)(1.0L / ((long double)q.x * q.y * q.z));
     // pass 2, every block, with the window spectrum recomputed instead of kept
     // (checkpointing: one block's spectra in memory at a time)
@@ -1881,22 +1901,30 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         MVD_CHECK(launch_z(sp, A, nullptr, 1.f, 0, q.z, st, -1.f));
         w_plan = nullptr;
       }
-      // R^ = FFT(residual of this block's effective region, zero elsewhere in the
-      // window); the residual image itself when that region is the whole image
-      const bool whole = g.eff_offset.x == 0 && g.eff_offset.y == 0 && g.eff_offset.z == 0 &&
-                         g.eff_size.x == n.x && g.eff_size.y == n.y && g.eff_size.z == n.z;
-      const float* rsrc = resid;
-      if (!whole) {
-        MVD_CHECK(launch_crop(s->g_crop.as<float>(), resid, n, g.eff_offset, g.eff_size, st));
-        rsrc = s->g_crop.as<float>();
+      if (fuse_r) {
+        // the residual rows are x-transformed already; rows outside the
+        // effective box are read as zero by the y pass
+        const Int3 lo = g.eff_local;
+        const Int3 hi = {lo.x + g.eff_size.x, lo.y + g.eff_size.y, lo.z + g.eff_size.z};
+        MVD_CHECK(launch_y_fast_window(sp, Cs, 0, q.z, 0, q.y, -1.f, lo, hi, st));
+      } else {
+        // R^ = FFT(residual of this block's effective region, zero elsewhere in
+        // the window); the residual image itself when that region is the image
+        const bool whole = g.eff_offset.x == 0 && g.eff_offset.y == 0 && g.eff_offset.z == 0 &&
+                           g.eff_size.x == n.x && g.eff_size.y == n.y && g.eff_size.z == n.z;
+        const float* rsrc = resid;
+        if (!whole) {
+          MVD_CHECK(launch_crop(s->g_crop.as<float>(), resid, n, g.eff_offset, g.eff_size, st));
+          rsrc = s->g_crop.as<float>();
+        }
+        ConvGeom gc = g;
+        gc.src = g.eff_size;
+        gc.off = {-g.eff_local.x, -g.eff_local.y, -g.eff_local.z};
+        MVD_CHECK(launch_xy_forward_passes(sp, gc, rsrc, kGatherZero, Cs, st));
+        // R^(0) = the residual's sum, in double
+        MVD_CHECK(launch_sum_to(rsrc, (int64_t)g.eff_size.x * g.eff_size.y * g.eff_size.z, s->g_rsum.as<double>(),
+                                kRsumParts, rsum, st));
       }
-      ConvGeom gc = g;
-      gc.src = g.eff_size;
-      gc.off = {-g.eff_local.x, -g.eff_local.y, -g.eff_local.z};
-      MVD_CHECK(launch_xy_forward_passes(sp, gc, rsrc, kGatherZero, Cs, st));
-      // R^(0) = the residual's sum, in double
-      MVD_CHECK(launch_sum_to(rsrc, (int64_t)g.eff_size.x * g.eff_size.y * g.eff_size.z, s->g_rsum.as<double>(),
-                              kRsumParts, rsum, st));
       // the adjoint's output box: the block window
       ConvGeom gw{};
       gw.src = g.win;
@@ -1974,19 +2002,18 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       }
     }
   }
-  for (int v = 0; v < s->nviews; ++v)
-    if (kofs[v + 1] > kofs[v])
-      MVD_CHECK(hipMemcpyAsync(grad_psf[v], s->g_kgrad.as<float>() + kofs[v], sizeof(float) * (kofs[v + 1] - kofs[v]),
-                               hipMemcpyDeviceToHost, st));
-  std::vector<double> sums(2 * s->nviews);
-  MVD_CHECK(hipMemcpyAsync(sums.data(), s->g_nll.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st));
   if (want_grad_psi) {
     MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
     MVD_CHECK(launch_to_float(s->g_acc.as<double>(), s->g_gpsi.as<float>(), s->n, st));
   }
+  std::vector<double> host((out_bytes + sizeof(double) - 1) / sizeof(double));
+  MVD_CHECK(hipMemcpyAsync(host.data(), s->g_out.p, out_bytes, hipMemcpyDeviceToHost, st));
   MVD_CHECK(hipStreamSynchronize(st));
+  const float* hk = reinterpret_cast<const float*>(host.data() + 2 * s->nviews);
+  for (int v = 0; v < s->nviews; ++v)
+    if (kofs[v + 1] > kofs[v]) std::memcpy(grad_psf[v], hk + kofs[v], sizeof(float) * (kofs[v + 1] - kofs[v]));
   double total = 0.0;
-  for (int v = 0; v < s->nviews; ++v) total += sums[2 * v];
+  for (int v = 0; v < s->nviews; ++v) total += host[2 * v];
   if (nll) *nll = total;
   s->g_gpsi_ready = want_grad_psi != 0;
   return 0;
