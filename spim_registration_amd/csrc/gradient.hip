@@ -235,6 +235,73 @@ hipError_t launch_set_dc(float2* spec, const double* v, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// The same fold as a gather over the image voxels the window covers (its
+// direct box [a, b)): per voxel, the window positions whose mirror-single
+// image it is -- the direct one and, near a face the window crosses, the one
+// reflected at that face -- are summed in double, then
+//   mode 0: acc += s;  mode 1: acc = s;  mode 2: out = (float)(acc + s);
+//   mode 3: out = (float) s.
+// No atomics: every voxel is written by one thread.  Needs a window that
+// extends past each image face by less than n - 1 (one reflection per face).
+__device__ __forceinline__ int fold_preimages(int p, int lo, int wd, int n, int* w) {
+  int c = 0;
+  w[c++] = p - lo;  // the direct position (inside [0, wd) on the direct box)
+  const int w1 = -p - lo, w2 = 2 * (n - 1) - p - lo;
+  if (p > 0 && w1 >= 0 && w1 < wd) w[c++] = w1;
+  if (p < n - 1 && w2 >= 0 && w2 < wd) w[c++] = w2;
+  return c;
+}
+
+__global__ __launch_bounds__(kGThreads) void k_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, Int3 a, Int3 b,
+                                                          double* acc, float* out, int mode) {
+  const int bx = b.x - a.x, by = b.y - a.y;
+  const int64_t total = (int64_t)bx * by * (b.z - a.z);
+  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
+    const int px = a.x + (int)(i % bx), py = a.y + (int)((i / bx) % by), pz = a.z + (int)(i / ((int64_t)bx * by));
+    int wx[3], wy[3], wz[3];
+    const int cx = fold_preimages(px, lo.x, wd.x, n.x, wx);
+    const int cy = fold_preimages(py, lo.y, wd.y, n.y, wy);
+    const int cz = fold_preimages(pz, lo.z, wd.z, n.z, wz);
+    double s = 0.0;
+    for (int k = 0; k < cz; ++k)
+      for (int j = 0; j < cy; ++j) {
+        const int64_t row = (int64_t)wd.x * (wy[j] + (int64_t)wd.y * wz[k]);
+        for (int l = 0; l < cx; ++l) s += (double)win[row + wx[l]];
+      }
+    const int64_t p = px + (int64_t)n.x * (py + (int64_t)n.y * pz);
+    if (mode == 0) acc[p] += s;
+    else if (mode == 1) acc[p] = s;
+    else if (mode == 2) out[p] = (float)(acc[p] + s);
+    else out[p] = (float)s;
+  }
+}
+
+bool fold_gather_ok(Int3 wd, Int3 lo, Int3 n) {
+  auto axis = [](int w, int l, int m) {
+    const int below = std::max(0, -l), above = std::max(0, l + w - m);
+    return m >= 2 && below < m - 1 && above < m - 1 && l < m && l + w > 0;
+  };
+  return axis(wd.x, lo.x, n.x) && axis(wd.y, lo.y, n.y) && axis(wd.z, lo.z, n.z);
+}
+
+bool fold_gather_covers(Int3 wd, Int3 lo, Int3 n) {
+  return lo.x <= 0 && lo.y <= 0 && lo.z <= 0 && lo.x + wd.x >= n.x && lo.y + wd.y >= n.y && lo.z + wd.z >= n.z;
+}
+
+hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, float* out, int mode,
+                              hipStream_t stream) {
+  if (!fold_gather_ok(wd, lo, n) || mode < 0 || mode > 3) return hipErrorInvalidValue;
+  const Int3 a = {std::max(0, lo.x), std::max(0, lo.y), std::max(0, lo.z)};
+  const Int3 b = {std::min(n.x, lo.x + wd.x), std::min(n.y, lo.y + wd.y), std::min(n.z, lo.z + wd.z)};
+  const int64_t total = (int64_t)(b.x - a.x) * (b.y - a.y) * (b.z - a.z);
+  const int64_t blocks = std::min<int64_t>((total + kGThreads - 1) / kGThreads, 16384);
+  if (blocks <= 0) return hipSuccess;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_fold_gather, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, a, b, acc, out,
+                     mode);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kGThreads) void k_to_float(const double* src, float* dst, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads)
     dst[i] = (float)src[i];

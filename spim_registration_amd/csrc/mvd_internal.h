@@ -382,6 +382,14 @@ hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scal
                                    hipStream_t stream);
 hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size, hipStream_t stream);
 hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream);
+// The same fold as a gather over the image voxels (no atomics), for windows
+// that extend past each image face by less than n - 1 (fold_gather_ok):
+// mode 0: acc += fold; 1: acc = fold; 2: out = (float)(acc + fold); 3: out = (float) fold.
+bool fold_gather_ok(Int3 wd, Int3 lo, Int3 n);
+// The window's direct box is the whole image.
+bool fold_gather_covers(Int3 wd, Int3 lo, Int3 n);
+hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, float* out, int mode,
+                              hipStream_t stream);
 // acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
 hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
 // *out = sum of the partials part[0, n) (one workgroup, a fixed order).

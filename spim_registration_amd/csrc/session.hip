@@ -1755,10 +1755,42 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(staged_h2d(s->g_psi.p, psi, sizeof(float) * s->n, st));
     psrc = s->g_psi.as<float>();
   }
+  // fold_once: every view has one block window covering the image, folded by
+  // the gather kernel: the first view writes the accumulator, the last one
+  // writes the float gradient (no clearing pass, no conversion pass)
+  bool fold_once = want_grad_psi != 0;
+  for (int v = 0; fold_once && v < s->nviews; ++v) {
+    const auto& view = s->views[v];
+    Int3 wd, lo;
+    if (s->prepared && view.plan) {
+      if (view.blocks.size() != 1) {
+        fold_once = false;
+        break;
+      }
+      wd = view.blocks[0].win;
+      lo = view.blocks[0].off;
+    } else {
+      wd = {n.x + view.kd.x - 1, n.y + view.kd.y - 1, n.z + view.kd.z - 1};
+      lo = {-view.kd.x / 2, -view.kd.y / 2, -view.kd.z / 2};
+    }
+    fold_once = fold_gather_ok(wd, lo, n) && fold_gather_covers(wd, lo, n);
+  }
   if (want_grad_psi) {
     MVD_CHECK(s->g_acc.ensure(sizeof(double) * s->n));
-    MVD_CHECK(launch_zero_double(s->g_acc.as<double>(), s->n, st));
+    MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
+    if (!fold_once) MVD_CHECK(launch_zero_double(s->g_acc.as<double>(), s->n, st));
   }
+  auto fold = [&](int v, const ConvGeom& g) -> hipError_t {
+    if (fold_once) {
+      if (!fold_gather_ok(g.win, g.off, n) || !fold_gather_covers(g.win, g.off, n)) return hipErrorInvalidValue;
+      const int mode = s->nviews == 1 ? 3 : v == 0 ? 1 : v == s->nviews - 1 ? 2 : 0;
+      return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), s->g_gpsi.as<float>(),
+                                mode, st);
+    }
+    if (fold_gather_ok(g.win, g.off, n))
+      return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), nullptr, 0, st);
+    return launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st);
+  };
   float* resid = s->d_ratio.as<float>();
   // the views' NLL sums and PSF gradients in one device buffer, copied to the
   // host in one transfer after the last view (a copy to pageable memory waits
@@ -1963,7 +1995,7 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
           es.mode = kEpiStore;
           es.dst = s->g_box.as<float>();
           MVD_CHECK(launch_yx_inverse_passes(sp, gw, Cs, es, st, nullptr));
-          MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
+          MVD_CHECK(fold(v, g));
         }
         continue;
       }
@@ -1998,14 +2030,12 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         es.mode = kEpiStore;
         es.dst = s->g_box.as<float>();
         MVD_CHECK(launch_yx_inverse_passes(sp, gw, B, es, st, nullptr));
-        MVD_CHECK(launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st));
+        MVD_CHECK(fold(v, g));
       }
     }
   }
-  if (want_grad_psi) {
-    MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
+  if (want_grad_psi && !fold_once)
     MVD_CHECK(launch_to_float(s->g_acc.as<double>(), s->g_gpsi.as<float>(), s->n, st));
-  }
   std::vector<double> host((out_bytes + sizeof(double) - 1) / sizeof(double));
   MVD_CHECK(hipMemcpyAsync(host.data(), s->g_out.p, out_bytes, hipMemcpyDeviceToHost, st));
   MVD_CHECK(hipStreamSynchronize(st));
