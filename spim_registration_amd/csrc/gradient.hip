@@ -123,15 +123,17 @@ hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 s
 // positions outside the image on some axis (the mirrored shell of a window
 // that crosses an image face) can share voxels with each other and with the
 // direct box: a second launch, after the first on the same stream, adds them
-// with FP64 atomics (their order, and with it the last bits, is not fixed).
+// with atomics (their order, and with it the last bits, is not fixed).  This
+// path serves windows that reach past an image face by n - 1 or more (several
+// reflections); the others take the gather below.
 __global__ __launch_bounds__(kGThreads) void k_fold_direct(const float* win, Int3 wd, Int3 lo, Int3 n, Int3 da,
-                                                          Int3 db, double* acc) {
+                                                          Int3 db, float* acc) {
   const int bx = db.x - da.x, by = db.y - da.y;
   const int64_t total = (int64_t)bx * by * (db.z - da.z);
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
     const int x = da.x + (int)(i % bx), y = da.y + (int)((i / bx) % by), z = da.z + (int)(i / ((int64_t)bx * by));
     const int64_t p = (lo.x + x) + (int64_t)n.x * ((lo.y + y) + (int64_t)n.y * (lo.z + z));
-    acc[p] += (double)win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)];
+    acc[p] += win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)];
   }
 }
 
@@ -145,7 +147,7 @@ struct ShellBoxes {
 };
 
 __global__ __launch_bounds__(kGThreads) void k_fold_mirrored(const float* win, Int3 wd, Int3 lo, Int3 n,
-                                                            ShellBoxes sb, double* acc) {
+                                                            ShellBoxes sb, float* acc) {
   const int64_t total = sb.prefix[sb.count];
   for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
     int k = 0;
@@ -155,11 +157,11 @@ __global__ __launch_bounds__(kGThreads) void k_fold_mirrored(const float* win, I
     const int bx = b.x - a.x, by = b.y - a.y;
     const int x = a.x + (int)(j % bx), y = a.y + (int)((j / bx) % by), z = a.z + (int)(j / ((int64_t)bx * by));
     const int px = mirror_single(lo.x + x, n.x), py = mirror_single(lo.y + y, n.y), pz = mirror_single(lo.z + z, n.z);
-    unsafeAtomicAdd(acc + (px + (int64_t)n.x * (py + (int64_t)n.y * pz)), (double)win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)]);
+    unsafeAtomicAdd(acc + (px + (int64_t)n.x * (py + (int64_t)n.y * pz)), win[x + (int64_t)wd.x * (y + (int64_t)wd.y * z)]);
   }
 }
 
-hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream) {
+hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, float* acc, hipStream_t stream) {
   const int64_t total = (int64_t)wd.x * wd.y * wd.z;
   if (total <= 0) return hipSuccess;
   const Int3 da = {std::max(0, -lo.x), std::max(0, -lo.y), std::max(0, -lo.z)};
@@ -238,11 +240,11 @@ hipError_t launch_set_dc(float2* spec, const double* v, hipStream_t stream) {
 // The same fold as a gather over the image voxels the window covers (its
 // direct box [a, b)): per voxel, the window positions whose mirror-single
 // image it is -- the direct one and, near a face the window crosses, the one
-// reflected at that face -- are summed in double, then
-//   mode 0: acc += s;  mode 1: acc = s;  mode 2: out = (float)(acc + s);
-//   mode 3: out = (float) s.
-// No atomics: every voxel is written by one thread.  Needs a window that
-// extends past each image face by less than n - 1 (one reflection per face).
+// reflected at that face -- are summed, then added to acc (add) or written
+// (!add).  No atomics: every voxel is written by one thread.  Needs a window
+// that extends past each image face by less than n - 1 (one reflection per
+// face).  One workgroup per 256 voxels of an image row: the y and z
+// preimages are the workgroup's.
 __device__ __forceinline__ int fold_preimages(int p, int lo, int wd, int n, int* w) {
   int c = 0;
   w[c++] = p - lo;  // the direct position (inside [0, wd) on the direct box)
@@ -253,27 +255,22 @@ __device__ __forceinline__ int fold_preimages(int p, int lo, int wd, int n, int*
 }
 
 __global__ __launch_bounds__(kGThreads) void k_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, Int3 a, Int3 b,
-                                                          double* acc, float* out, int mode) {
-  const int bx = b.x - a.x, by = b.y - a.y;
-  const int64_t total = (int64_t)bx * by * (b.z - a.z);
-  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kGThreads) {
-    const int px = a.x + (int)(i % bx), py = a.y + (int)((i / bx) % by), pz = a.z + (int)(i / ((int64_t)bx * by));
-    int wx[3], wy[3], wz[3];
-    const int cx = fold_preimages(px, lo.x, wd.x, n.x, wx);
-    const int cy = fold_preimages(py, lo.y, wd.y, n.y, wy);
-    const int cz = fold_preimages(pz, lo.z, wd.z, n.z, wz);
-    double s = 0.0;
-    for (int k = 0; k < cz; ++k)
-      for (int j = 0; j < cy; ++j) {
-        const int64_t row = (int64_t)wd.x * (wy[j] + (int64_t)wd.y * wz[k]);
-        for (int l = 0; l < cx; ++l) s += (double)win[row + wx[l]];
-      }
-    const int64_t p = px + (int64_t)n.x * (py + (int64_t)n.y * pz);
-    if (mode == 0) acc[p] += s;
-    else if (mode == 1) acc[p] = s;
-    else if (mode == 2) out[p] = (float)(acc[p] + s);
-    else out[p] = (float)s;
-  }
+                                                          float* acc, int add) {
+  const int px = a.x + blockIdx.x * kGThreads + threadIdx.x;
+  const int py = a.y + blockIdx.y, pz = a.z + blockIdx.z;
+  if (px >= b.x) return;
+  int wx[3], wy[3], wz[3];
+  const int cy = fold_preimages(py, lo.y, wd.y, n.y, wy);
+  const int cz = fold_preimages(pz, lo.z, wd.z, n.z, wz);
+  const int cx = fold_preimages(px, lo.x, wd.x, n.x, wx);
+  float s = 0.f;
+  for (int k = 0; k < cz; ++k)
+    for (int j = 0; j < cy; ++j) {
+      const float* row = win + (int64_t)wd.x * (wy[j] + (int64_t)wd.y * wz[k]);
+      for (int l = 0; l < cx; ++l) s += row[wx[l]];
+    }
+  float* p = acc + px + (int64_t)n.x * (py + (int64_t)n.y * pz);
+  *p = add ? *p + s : s;
 }
 
 bool fold_gather_ok(Int3 wd, Int3 lo, Int3 n) {
@@ -288,43 +285,15 @@ bool fold_gather_covers(Int3 wd, Int3 lo, Int3 n) {
   return lo.x <= 0 && lo.y <= 0 && lo.z <= 0 && lo.x + wd.x >= n.x && lo.y + wd.y >= n.y && lo.z + wd.z >= n.z;
 }
 
-hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, float* out, int mode,
-                              hipStream_t stream) {
-  if (!fold_gather_ok(wd, lo, n) || mode < 0 || mode > 3) return hipErrorInvalidValue;
+hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, float* acc, bool add, hipStream_t stream) {
+  if (!fold_gather_ok(wd, lo, n)) return hipErrorInvalidValue;
   const Int3 a = {std::max(0, lo.x), std::max(0, lo.y), std::max(0, lo.z)};
   const Int3 b = {std::min(n.x, lo.x + wd.x), std::min(n.y, lo.y + wd.y), std::min(n.z, lo.z + wd.z)};
-  const int64_t total = (int64_t)(b.x - a.x) * (b.y - a.y) * (b.z - a.z);
-  const int64_t blocks = std::min<int64_t>((total + kGThreads - 1) / kGThreads, 16384);
-  if (blocks <= 0) return hipSuccess;
+  if (b.x <= a.x || b.y <= a.y || b.z <= a.z) return hipSuccess;
+  if (b.y - a.y > 65535 || b.z - a.z > 65535) return hipErrorInvalidValue;
   (void)hipGetLastError();
-  hipLaunchKernelGGL(k_fold_gather, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, win, wd, lo, n, a, b, acc, out,
-                     mode);
-  return hipGetLastError();
-}
-
-__global__ __launch_bounds__(kGThreads) void k_to_float(const double* src, float* dst, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads)
-    dst[i] = (float)src[i];
-}
-
-hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream) {
-  const int64_t blocks = std::min<int64_t>((n + kGThreads - 1) / kGThreads, 16384);
-  if (blocks == 0) return hipSuccess;
-  (void)hipGetLastError();
-  hipLaunchKernelGGL(k_to_float, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, src, dst, n);
-  return hipGetLastError();
-}
-
-__global__ __launch_bounds__(kGThreads) void k_zero_double(double* dst, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)kGThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kGThreads)
-    dst[i] = 0.0;
-}
-
-hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream) {
-  const int64_t blocks = std::min<int64_t>((n + kGThreads - 1) / kGThreads, 16384);
-  if (blocks == 0) return hipSuccess;
-  (void)hipGetLastError();
-  hipLaunchKernelGGL(k_zero_double, dim3((unsigned)blocks), dim3(kGThreads), 0, stream, dst, n);
+  const dim3 grid((b.x - a.x + kGThreads - 1) / kGThreads, b.y - a.y, b.z - a.z);
+  hipLaunchKernelGGL(k_fold_gather, grid, dim3(kGThreads), 0, stream, win, wd, lo, n, a, b, acc, add ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -381,24 +381,22 @@ hipError_t launch_extract_psf_grad_shifted(const float* box, Int3 kd, float scal
 hipError_t launch_extract_psf_grad(const float* box, Int3 q, Int3 kd, float scale, float* grad, int accumulate,
                                    hipStream_t stream);
 hipError_t launch_crop(float* dst, const float* src, Int3 n, Int3 origin, Int3 size, hipStream_t stream);
-hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, hipStream_t stream);
+// acc[mirror(lo + u)] += win[u] over the window positions u in [0, wd) (the
+// adjoint of the mirror-single window gather; atomics for the mirrored shell).
+hipError_t launch_fold_window(const float* win, Int3 wd, Int3 lo, Int3 n, float* acc, hipStream_t stream);
 // The same fold as a gather over the image voxels (no atomics), for windows
-// that extend past each image face by less than n - 1 (fold_gather_ok):
-// mode 0: acc += fold; 1: acc = fold; 2: out = (float)(acc + fold); 3: out = (float) fold.
+// that extend past each image face by less than n - 1 (fold_gather_ok): the
+// fold is added to acc (add) or written to it.
 bool fold_gather_ok(Int3 wd, Int3 lo, Int3 n);
 // The window's direct box is the whole image.
 bool fold_gather_covers(Int3 wd, Int3 lo, Int3 n);
-hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, double* acc, float* out, int mode,
-                              hipStream_t stream);
-// acc[p] += sum of win[u + c] over window positions u in [-c, n + c) with mirror(u) = p.
-hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t stream);
+hipError_t launch_fold_gather(const float* win, Int3 wd, Int3 lo, Int3 n, float* acc, bool add, hipStream_t stream);
 // *out = sum of the partials part[0, n) (one workgroup, a fixed order).
 hipError_t launch_reduce_sum(const double* part, int n, double* out, hipStream_t stream);
 // *out = sum of a[0, n) in double, through nparts partials in part.
 hipError_t launch_sum_to(const float* a, int64_t n, double* part, int nparts, double* out, hipStream_t stream);
 // spec[0] = (float) *v (the zero-frequency value of a spectrum).
 hipError_t launch_set_dc(float2* spec, const double* v, hipStream_t stream);
-hipError_t launch_zero_double(double* dst, int64_t n, hipStream_t stream);
 
 // Resampling of one view into the bounding box with its blending weight
 // (transform.hip).  inv: row-packed 3x4 inverse affine map (bounding-box

@@ -632,7 +632,7 @@ struct mvd_session {
   DeviceBuf d_views, d_weights, d_wsum, d_psi, d_ratio, d_spec, d_part_sum, d_part_max, d_stats, d_tmp;
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
-  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_acc, g_part_sum, g_part_max;
+  DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_part_sum, g_part_max;
   DeviceBuf g_out;  // per call: the views' NLL sums (double) and PSF gradients (float), copied out at once
   DeviceBuf g_rsum;  // partials and the double sum of one block's residual
   DeviceBuf g_crop, g_gpsi;
@@ -1755,9 +1755,9 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     MVD_CHECK(staged_h2d(s->g_psi.p, psi, sizeof(float) * s->n, st));
     psrc = s->g_psi.as<float>();
   }
-  // fold_once: every view has one block window covering the image, folded by
-  // the gather kernel: the first view writes the accumulator, the last one
-  // writes the float gradient (no clearing pass, no conversion pass)
+  // dL/dpsi is accumulated in g_gpsi.  fold_once: every view has one block
+  // window covering the image, folded by the gather kernel, and the first
+  // view writes the gradient instead of adding to it (no clearing pass)
   bool fold_once = want_grad_psi != 0;
   for (int v = 0; fold_once && v < s->nviews; ++v) {
     const auto& view = s->views[v];
@@ -1775,21 +1775,19 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
     }
     fold_once = fold_gather_ok(wd, lo, n) && fold_gather_covers(wd, lo, n);
   }
+  float* gpsi = nullptr;
   if (want_grad_psi) {
-    MVD_CHECK(s->g_acc.ensure(sizeof(double) * s->n));
     MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
-    if (!fold_once) MVD_CHECK(launch_zero_double(s->g_acc.as<double>(), s->n, st));
+    gpsi = s->g_gpsi.as<float>();
+    if (!fold_once) MVD_CHECK(launch_fill(gpsi, 0.f, s->n, st));
   }
   auto fold = [&](int v, const ConvGeom& g) -> hipError_t {
     if (fold_once) {
       if (!fold_gather_ok(g.win, g.off, n) || !fold_gather_covers(g.win, g.off, n)) return hipErrorInvalidValue;
-      const int mode = s->nviews == 1 ? 3 : v == 0 ? 1 : v == s->nviews - 1 ? 2 : 0;
-      return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), s->g_gpsi.as<float>(),
-                                mode, st);
+      return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, gpsi, v > 0, st);
     }
-    if (fold_gather_ok(g.win, g.off, n))
-      return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), nullptr, 0, st);
-    return launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, s->g_acc.as<double>(), st);
+    if (fold_gather_ok(g.win, g.off, n)) return launch_fold_gather(s->g_box.as<float>(), g.win, g.off, n, gpsi, true, st);
+    return launch_fold_window(s->g_box.as<float>(), g.win, g.off, n, gpsi, st);
   };
   float* resid = s->d_ratio.as<float>();
   // the views' NLL sums and PSF gradients in one device buffer, copied to the
@@ -2034,8 +2032,6 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       }
     }
   }
-  if (want_grad_psi && !fold_once)
-    MVD_CHECK(launch_to_float(s->g_acc.as<double>(), s->g_gpsi.as<float>(), s->n, st));
   std::vector<double> host((out_bytes + sizeof(double) - 1) / sizeof(double));
   MVD_CHECK(hipMemcpyAsync(host.data(), s->g_out.p, out_bytes, hipMemcpyDeviceToHost, st));
   MVD_CHECK(hipStreamSynchronize(st));
