@@ -227,6 +227,8 @@ struct ZGradParams {
   const float2* twx;  // exp(-2 pi i m / q.x), m < q.x
   const float2* twy;  // exp(-2 pi i m / q.y), m < q.y
   const double* rsum;  // R^(0) from the residual's double sum (nullable)
+  float2* adj;         // the adjoint rows' destination (nullable: spec, in place)
+  int adj_add;         // add the adjoint rows to adj instead of storing them
 };
 
 __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
@@ -279,9 +281,14 @@ __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
   __syncthreads();
   if (adj) lds_fft(buf, lp, B, P.plan, tw, 1, 1.f);
   if (corr && P.mode == 1) lds_fft(cbuf, lp, B, P.plan, tw, 1, 1.f);
+  float2* adj_out = P.adj ? P.adj : P.spec;
   for (int idx = threadIdx.x; idx < (P.zhi - P.zlo) * B; idx += kThreads) {
     const int z = P.zlo + idx / B, b = idx % B;
-    if (c0 + b < P.hx) gmem(P.spec)[z * plane + col + b] = buf[z * lp + b];
+    if (c0 + b < P.hx) {
+      const int64_t o = z * plane + col + b;
+      const float2 x = buf[z * lp + b];
+      gmem(adj_out)[o] = P.adj_add ? cadd(gmem(adj_out)[o], x) : x;
+    }
   }
   if (corr && P.mode == 1) {
     for (int idx = threadIdx.x; idx < P.kz * B; idx += kThreads) {
@@ -504,12 +511,14 @@ bool z_grad_supported(const ShapePlan& sp) {
 
 hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
                          int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
-                         hipStream_t stream) {
+                         float2* adj, bool adj_add, hipStream_t stream) {
   if (!z_grad_supported(sp) || zlo < 0 || zhi > sp.q.z || kz < 0 || kz > sp.q.z) return hipErrorInvalidValue;
   if (fast_size(sp.q.z))
-    return launch_z_fast_grad(sp, spec, what, khat, corr, mode, first, shift, zlo, zhi, kz, rsum, stream);
+    return launch_z_fast_grad(sp, spec, what, khat, corr, mode, first, shift, zlo, zhi, kz, rsum, adj, adj_add, stream);
   ZGradParams P{};
   P.rsum = rsum;
+  P.adj = adj;
+  P.adj_add = adj_add ? 1 : 0;
   P.spec = spec;
   P.what = what;
   P.khat = khat;

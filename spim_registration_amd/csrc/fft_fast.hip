@@ -328,6 +328,8 @@ struct ZGradFastParams {
   Int3 shift, q;
   int hx, pitch, zlo, zhi, kz;
   const double* rsum;  // R^(0) from the residual's double sum (nullable)
+  float2* adj;         // the adjoint rows' destination (nullable: spec, in place)
+  int adj_add;         // add the adjoint rows to adj instead of storing them
 };
 
 template <int N>
@@ -412,12 +414,16 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
       }
     fft_reverse_order<N>(v, lds, jt, tw, 1, 1.f, addr);
     if (valid) {
+      float2* out = P.adj ? P.adj : P.spec;
 #pragma unroll
       for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
         for (int t = 0; t < S::R0; ++t) {
           const int z = elem_index<N, E, S::R0>(jt, b, t);
-          if (z >= P.zlo && z < P.zhi) gmem(P.spec)[z * plane + off] = v[b * S::R0 + t];
+          if (z >= P.zlo && z < P.zhi) {
+            const int o = z * plane + off;
+            gmem(out)[o] = P.adj_add ? cadd(gmem(out)[o], v[b * S::R0 + t]) : v[b * S::R0 + t];
+          }
         }
     }
   }
@@ -425,9 +431,9 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
 
 hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
                               int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
-                              hipStream_t stream) {
+                              float2* adj, bool adj_add, hipStream_t stream) {
   ZGradFastParams P{spec, what, khat, corr, sp.tw_z, sp.tw_x, sp.tw_y, mode, first, shift, sp.q,
-                    sp.hx, sp.pitch, zlo, zhi, kz, rsum};
+                    sp.hx, sp.pitch, zlo, zhi, kz, rsum, adj, adj_add ? 1 : 0};
   (void)hipGetLastError();
 #define MVD_ZG(N)                                                                               \
   case N: {                                                                                     \

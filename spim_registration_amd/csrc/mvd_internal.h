@@ -230,14 +230,17 @@ hipError_t launch_z(const ShapePlan& sp, float2* spec, const float2* kspec, floa
 // y-transformed residual in spec; adjoint R^ conj(P^) inverted into rows
 // [zlo, zhi) of spec; correlation conj(R^) W^ with the lag shift `shift`
 // accumulated into corr (mode 0, first: overwrite) or inverted along z into
-// rows [0, kz) of corr (mode 1).  rsum (nullable): the residual's sum in
+// rows [0, kz) of corr (mode 1); adj (nullable): the adjoint rows go there
+// instead (added to what it holds when adj_add: the views' adjoints are
+// linear and share the window, so their sum is transformed once).  rsum
+// (nullable): the residual's sum in
 // double, which replaces the zero-frequency value R^(0) of the transform (an
 // FP32 transform of a residual of mixed signs loses that value's low bits,
 // and the correlation multiplies it by the window's sum).
 bool z_grad_supported(const ShapePlan& sp);
 hipError_t launch_z_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
                          int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
-                         hipStream_t stream);
+                         float2* adj, bool adj_add, hipStream_t stream);
 // Returns the number of workgroups (for the update statistics) in *nwg.
 hipError_t launch_x_inverse(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,
                             hipStream_t stream, int* nwg);
@@ -292,7 +295,7 @@ hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec,
                          hipStream_t stream, float dir, float2* aux = nullptr, const float2* pre = nullptr);
 hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* what, const float2* khat, float2* corr,
                               int mode, int first, Int3 shift, int zlo, int zhi, int kz, const double* rsum,
-                              hipStream_t stream);
+                              float2* adj, bool adj_add, hipStream_t stream);
 hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const float* src, int gather, float2* spec,
                                  hipStream_t stream);
 hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const float2* spec, const EpilogueArgs& e,

@@ -633,6 +633,7 @@ struct mvd_session {
   DeviceBuf d_fi_sum, d_fi_cnt;
   // likelihood gradient workspace
   DeviceBuf g_spec_a, g_spec_b, g_spec_c, g_spec_d, g_box, g_khat, g_psi, g_part_sum, g_part_max;
+  DeviceBuf g_adj;  // the views' adjoint spectra summed (views sharing one window)
   DeviceBuf g_out;  // per call: the views' NLL sums (double) and PSF gradients (float), copied out at once
   DeviceBuf g_rsum;  // partials and the double sum of one block's residual
   DeviceBuf g_crop, g_gpsi;
@@ -1758,10 +1759,16 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
   // dL/dpsi is accumulated in g_gpsi.  fold_once: every view has one block
   // window covering the image, folded by the gather kernel, and the first
   // view writes the gradient instead of adding to it (no clearing pass)
-  bool fold_once = want_grad_psi != 0;
+  // sum_adj: besides, all views share that window and its FFT shape, so their
+  // adjoint spectra (linear in the residual) are summed along the views and
+  // transformed back and folded once
+  bool fold_once = want_grad_psi != 0, sum_adj = fold_once && g_grad_fused && g_grad_keep_w;
+  const ShapePlan* first_plan = nullptr;
+  Int3 first_wd{0, 0, 0}, first_lo{0, 0, 0};
   for (int v = 0; fold_once && v < s->nviews; ++v) {
     const auto& view = s->views[v];
     Int3 wd, lo;
+    const ShapePlan* pl = nullptr;
     if (s->prepared && view.plan) {
       if (view.blocks.size() != 1) {
         fold_once = false;
@@ -1769,12 +1776,25 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
       }
       wd = view.blocks[0].win;
       lo = view.blocks[0].off;
+      pl = view.plan;
     } else {
       wd = {n.x + view.kd.x - 1, n.y + view.kd.y - 1, n.z + view.kd.z - 1};
       lo = {-view.kd.x / 2, -view.kd.y / 2, -view.kd.z / 2};
+      pl = s->plans.get(fft_shape_for(wd));
     }
     fold_once = fold_gather_ok(wd, lo, n) && fold_gather_covers(wd, lo, n);
+    if (v == 0) {
+      first_plan = pl;
+      first_wd = wd;
+      first_lo = lo;
+    }
+    sum_adj = sum_adj && pl && pl == first_plan && z_grad_supported(*pl) && wd.x == first_wd.x &&
+              wd.y == first_wd.y && wd.z == first_wd.z && lo.x == first_lo.x && lo.y == first_lo.y &&
+              lo.z == first_lo.z;
   }
+  sum_adj = sum_adj && fold_once;
+  if (sum_adj) MVD_CHECK(s->g_adj.ensure(sizeof(float2) * first_plan->spectrum_elems()));
+  ConvGeom adj_g{};  // the shared window (sum_adj)
   float* gpsi = nullptr;
   if (want_grad_psi) {
     MVD_CHECK(s->g_gpsi.ensure(sizeof(float) * s->n));
@@ -1972,8 +1992,11 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         float2* D = own_d ? s->g_spec_d.as<float2>() : B;
         MVD_REQUIRE(!want_psf || !own_d || s->g_spec_d.bytes >= sizeof(float2) * hs, "internal: correlation spectrum");
         const Int3 shift = {(k.x - 1) / 2, (k.y - 1) / 2, (k.z - 1) / 2};
+        const bool adj_sum = sum_adj && keep_w;
+        MVD_REQUIRE(!sum_adj || keep_w, "internal: summed adjoint without a kept window spectrum");
         MVD_CHECK(launch_z_grad(sp, Cs, A, Kh, want_psf ? D : nullptr, keep_w ? 1 : 0, bi == 0, shift, 0,
-                                want_grad_psi ? g.win.z : 0, keep_w && want_psf ? k.z : 0, rsum, st));
+                                want_grad_psi ? g.win.z : 0, keep_w && want_psf ? k.z : 0, rsum,
+                                adj_sum ? s->g_adj.as<float2>() : nullptr, adj_sum && v > 0, st));
         if (want_psf && last) {
           if (!keep_w) MVD_CHECK(launch_z(sp, D, nullptr, 1.f, 0, q.z, st, 1.f));
           ConvGeom gk{};
@@ -1988,7 +2011,10 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
           MVD_CHECK(launch_yx_inverse_passes(sp, gk, D, es, st, nullptr));
           MVD_CHECK(launch_extract_psf_grad_shifted(s->g_box.as<float>(), k, scale, kgrad, st));
         }
-        if (want_grad_psi) {
+        if (want_grad_psi && adj_sum) {
+          adj_g = gw;
+          adj_g.off = g.off;
+        } else if (want_grad_psi) {
           EpilogueArgs es{};
           es.mode = kEpiStore;
           es.dst = s->g_box.as<float>();
@@ -2031,6 +2057,14 @@ int mvd_nll_grad_device(mvd_session* s, const float* psi, const float* const* ps
         MVD_CHECK(fold(v, g));
       }
     }
+  }
+  if (sum_adj) {
+    // the summed adjoint: y and x back over the window, then one fold
+    EpilogueArgs es{};
+    es.mode = kEpiStore;
+    es.dst = s->g_box.as<float>();
+    MVD_CHECK(launch_yx_inverse_passes(*first_plan, adj_g, s->g_adj.as<float2>(), es, st, nullptr));
+    MVD_CHECK(fold(0, adj_g));
   }
   std::vector<double> host((out_bytes + sizeof(double) - 1) / sizeof(double));
   MVD_CHECK(hipMemcpyAsync(host.data(), s->g_out.p, out_bytes, hipMemcpyDeviceToHost, st));
