@@ -67,11 +67,21 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_y_fast(YFastParams P) {
   y_fast_body<N>(P, blockIdx.x, blockIdx.y);
 }
 
+// tasks (optional): one (item, plane) per wpt workgroups -- only the planes
+// listed are transformed (the others are identical to a listed plane and read
+// from it by the z pass); wpt = the item's column groups.
 template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_y_fast_batch(const YFastParams* items, const int* prefix, const int* gx,
-                                                   int nitems) {
+                                                   int nitems, const int2* tasks, int wpt) {
   int item, bx, by;
-  batch_locate(prefix, gx, nitems, item, bx, by);
+  if (tasks) {
+    const int2 t = tasks[blockIdx.x / wpt];
+    item = t.x;
+    bx = blockIdx.x % wpt;
+    by = t.y - items[item].zlo;
+  } else {
+    batch_locate(prefix, gx, nitems, item, bx, by);
+  }
   const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
   y_fast_body<N>(item_params, bx, by);
 }
@@ -193,11 +203,13 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 // branched around, so the wait for the prefetch does not depend on how many
 // stores a lane made.
 //
-// TABLE: the items' input planes come through their plane tables (the
-// launch's tables, at most kZTabEntries entries, are copied to LDS first);
+// TAB: the items' input planes come through their plane tables (the
+// launch's tables, at most ztab_entries(N) entries, are copied to LDS first);
 // every table entry addresses a plane that no item of the launch overwrites,
-// so the items can run in any order.
-template <int N, bool TABLE>
+// so the items can run in any order.  TAB = 1 addresses the arena of spectra
+// with 32-bit buffer offsets (fewer than 2^31 bytes), TAB = 2 with 64-bit
+// addresses.
+template <int N, int TAB>
 __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
@@ -205,7 +217,8 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
   __shared__ float2 lds[N * LP];
-  __shared__ unsigned short ztab[TABLE ? kZTabEntries : 1];
+  constexpr bool TABLE = TAB != 0;
+  __shared__ unsigned short ztab[TABLE ? ztab_entries(N) : 1];
   const ZFastParams P0 = items[0];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P0.tw)[i];
@@ -229,15 +242,30 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
   const uint32_t off = (uint32_t)(store_col ? col : P0.hx - 1) * 8u;
   auto addr = [line](int i) { return i * LP + line; };
   auto load_line = [&](int it, const float2* spec, float2(&v)[E]) {
-    const BufRes r = TABLE ? buf_res(P0.arena, P0.arena_bytes) : buf_res(spec, bytes);
+    if constexpr (TAB == 2) {
+      // 64-bit: arena + plane index * plane + row + column
+      const auto base = gmem(P0.arena) + (size_t)(blockIdx.y * P0.pitch) + (off >> 3);
+      const size_t plane_elems = (size_t)P0.qy * P0.pitch;
 #pragma unroll
-    for (int b = 0; b < E / S::R0; ++b)
+      for (int b = 0; b < E / S::R0; ++b)
 #pragma unroll
-      for (int t = 0; t < S::R0; ++t) {
-        const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
-        const uint32_t zp = TABLE ? (uint32_t)ztab[it * N + z] : z;
-        v[b * S::R0 + t] = kStream ? buf_load2_nt(r, zp * plane + off, row) : buf_load2(r, zp * plane + off, row);
-      }
+        for (int t = 0; t < S::R0; ++t) {
+          const int z = elem_index<N, E, S::R0>(jt, b, t);
+          const auto ptr = base + (size_t)ztab[it * N + z] * plane_elems;
+          if constexpr (kStream) v[b * S::R0 + t] = stream_load(ptr);
+          else v[b * S::R0 + t] = *ptr;
+        }
+    } else {
+      const BufRes r = TABLE ? buf_res(P0.arena, P0.arena_bytes) : buf_res(spec, bytes);
+#pragma unroll
+      for (int b = 0; b < E / S::R0; ++b)
+#pragma unroll
+        for (int t = 0; t < S::R0; ++t) {
+          const uint32_t z = elem_index<N, E, S::R0>(jt, b, t);
+          const uint32_t zp = TABLE ? (uint32_t)ztab[it * N + z] : z;
+          v[b * S::R0 + t] = kStream ? buf_load2_nt(r, zp * plane + off, row) : buf_load2(r, zp * plane + off, row);
+        }
+    }
   };
   float2 nxt[E];
   load_line(0, P0.spec, nxt);
@@ -283,15 +311,16 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
   }
 }
 
-hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, bool table) {
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, int table) {
   if (nitems <= 0 || gx <= 0 || gy <= 0) return hipSuccess;
-  if (table && nitems * n > kZTabEntries) return hipErrorInvalidValue;
+  if (table < 0 || table > 2 || (table && nitems * n > ztab_entries(n))) return hipErrorInvalidValue;
   (void)hipGetLastError();
   const ZFastParams* it = static_cast<const ZFastParams*>(items);
   const dim3 grid(gx, gy);
-#define MVD_Z_LINES(NN)                                                                                        \
-  if (table) hipLaunchKernelGGL((k_z_fast_lines<NN, true>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems); \
-  else hipLaunchKernelGGL((k_z_fast_lines<NN, false>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems);
+#define MVD_Z_LINES(NN)                                                                                         \
+  if (table == 2) hipLaunchKernelGGL((k_z_fast_lines<NN, 2>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems);  \
+  else if (table == 1) hipLaunchKernelGGL((k_z_fast_lines<NN, 1>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems); \
+  else hipLaunchKernelGGL((k_z_fast_lines<NN, 0>), grid, dim3(Spec<NN>::NT), 0, stream, it, nitems);
   switch (n) {
     case 64: MVD_Z_LINES(64) break;
     case 128: MVD_Z_LINES(128) break;
@@ -629,11 +658,20 @@ __global__ __launch_bounds__(Spec<M>::NT) void k_x_forward_fast(XFwdFastParams P
   x_forward_fast_body<M, MODE>(P, blockIdx.x, blockIdx.y);
 }
 
+// tasks (optional): one (item, plane) per wpt = q.y / L workgroups, each
+// workgroup L rows of the plane; only the planes listed are transformed.
 template <int M, int MODE>
 __global__ __launch_bounds__(Spec<M>::NT) void k_x_forward_fast_batch(const XFwdFastParams* items, const int* prefix, const int* gx,
-                                                   int nitems) {
+                                                   int nitems, const int2* tasks, int wpt) {
   int item, bx, by;
-  batch_locate(prefix, gx, nitems, item, bx, by);
+  if (tasks) {
+    const int2 t = tasks[blockIdx.x / wpt];
+    item = t.x;
+    bx = t.y * wpt + blockIdx.x % wpt;
+    by = 0;
+  } else {
+    batch_locate(prefix, gx, nitems, item, bx, by);
+  }
   const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
   x_forward_fast_body<M, MODE>(item_params, bx, by);
 }
@@ -1004,7 +1042,8 @@ int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) 
 }
 
 hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
-                             int nwg, hipStream_t stream) {
+                             int nwg, hipStream_t stream, const int2* tasks, int wpt) {
+  if (tasks && (wpt <= 0 || (kind != kPassXForward && kind != kPassYForward))) return hipErrorInvalidValue;
   if (nwg <= 0) return hipSuccess;
   (void)hipGetLastError();
   const dim3 grid(nwg);
@@ -1016,12 +1055,12 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
     case kPassYInverse: {
       const YFastParams* it = static_cast<const YFastParams*>(items);
       switch (cls) {
-        case 64: hipLaunchKernelGGL(k_y_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems); break;
-        case 128: hipLaunchKernelGGL(k_y_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
-        case 144: hipLaunchKernelGGL(k_y_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems); break;
-        case 256: hipLaunchKernelGGL(k_y_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
-        case 288: hipLaunchKernelGGL(k_y_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems); break;
-        case 512: hipLaunchKernelGGL(k_y_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 64: hipLaunchKernelGGL(k_y_fast_batch<64>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 128: hipLaunchKernelGGL(k_y_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 144: hipLaunchKernelGGL(k_y_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 256: hipLaunchKernelGGL(k_y_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 288: hipLaunchKernelGGL(k_y_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 512: hipLaunchKernelGGL(k_y_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         default: return hipErrorInvalidValue;
       }
       break;
@@ -1043,7 +1082,7 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
       const XFwdFastParams* it = static_cast<const XFwdFastParams*>(items);
 #define MVD_BXF(M, MODE)                                                                                   \
   case M * 16 + MODE:                                                                                      \
-    hipLaunchKernelGGL((k_x_forward_fast_batch<M, MODE>), grid, block, 0, stream, it, prefix, gx, nitems); \
+    hipLaunchKernelGGL((k_x_forward_fast_batch<M, MODE>), grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); \
     break;
 #define MVD_BXF4(M) MVD_BXF(M, kGatherMirror) MVD_BXF(M, kGatherOne) MVD_BXF(M, kGatherZero) MVD_BXF(M, kGatherKernelWrap)
       switch (cls) {

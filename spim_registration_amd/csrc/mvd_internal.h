@@ -338,13 +338,18 @@ int build_generic_params(const PassDesc& d, void* out, int* gx, int* gy, size_t*
 int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls);
 hipError_t launch_batch_generic(int kind, int epi, const void* items, const int* prefix, const int* gx, int nitems,
                                 int nwg, size_t lds, hipStream_t stream);
+// tasks (x and y forward only, optional): one (item, plane) per wpt
+// workgroups; only the listed planes of the items are transformed.
 hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* prefix, const int* gx, int nitems,
-                             int nwg, hipStream_t stream);
+                             int nwg, hipStream_t stream, const int2* tasks = nullptr, int wpt = 0);
 // z passes of many blocks sharing one kernel spectrum and line grid (gx, gy);
 // table: the items' input planes come through their plane tables, at most
-// kZTabEntries entries for the launch.
-constexpr int kZTabEntries = 6144;
-hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, bool table);
+// ztab_entries(n) entries for the launch (1: the arena spans fewer than 2^31
+// bytes, 32-bit offsets; 2: 64-bit addresses).  The table lives in LDS beside
+// the lines: 512-point lines take up to 18 blocks of 512 planes, the shorter
+// ones 6144 entries (which keeps three 256-point workgroups per CU).
+constexpr int ztab_entries(int n) { return n >= 512 ? 9216 : 6144; }
+hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, hipStream_t stream, int table);
 // One workgroup of a fused forward launch with a task list: transform plane
 // `plane` of item `item` and store the spectrum into plane dest[d][1] of the
 // spectrum of item dest[d][0], d < ndest (identical planes of several block

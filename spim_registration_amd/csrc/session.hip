@@ -397,7 +397,9 @@ struct PassLaunch {
   DeviceBuf items, prefix, gx;
   DeviceBuf sched;  // fused forward: one PlaneTask per workgroup
   bool has_sched = false;
-  bool table = false;  // lines z pass reading its input planes through tables
+  int table = 0;  // lines z pass reading its input planes through tables (1: 32-bit, 2: 64-bit offsets)
+  DeviceBuf ptasks;  // separate x and y forward: (item, plane) per wpt workgroups
+  int wpt = 0;
 };
 
 // MVD_XCD_SCHEDULE=0 keeps the item-major order of the forward plane
@@ -585,7 +587,7 @@ bool plan_conv_planes(const std::vector<PassDesc>& fwd, const std::vector<PassDe
   }
   int max_slot = 0;
   for (int sl : slot) max_slot = std::max(max_slot, sl);
-  if ((int64_t)(max_slot + 1) * qz > 65535 || (int64_t)fwd.size() * qz > kZTabEntries) return false;
+  if ((int64_t)(max_slot + 1) * qz > 65535 || (int64_t)fwd.size() * qz > ztab_entries(qz)) return false;
   ztab->assign(fwd.size() * qz, 0);
   std::vector<OrderedTask> out;
   for (const PlaneGroup& g : plane_groups(items, g_plane_dedup)) {
@@ -604,6 +606,52 @@ bool plan_conv_planes(const std::vector<PassDesc>& fwd, const std::vector<PassDe
     }
   }
   *tasks = order_tasks(std::move(out));
+  return true;
+}
+
+// The same for the separate x and y passes (windows wider than the fused
+// plane kernels take): the x and y forward transform only the planes listed
+// in tasks, (item, plane) -- one per distinct plane that has an occurrence
+// outside its own block's output planes (read there through the tables by
+// every occurrence), and every occurrence of the others.
+bool plan_sep_planes(const std::vector<PassDesc>& fwd, const std::vector<PassDesc>& zp, const float2* arena,
+                     int64_t elems, std::vector<int2>* tasks, std::vector<unsigned short>* ztab) {
+  if (fwd.empty() || fwd.size() != zp.size() || !arena || elems <= 0) return false;
+  const int qz = fwd[0].g.q.z;
+  std::vector<const PassDesc*> items;
+  std::vector<int> slot;
+  for (size_t i = 0; i < fwd.size(); ++i) {
+    if (fwd[i].g.q.z != qz || fwd[i].spec != zp[i].spec) return false;
+    const int64_t off = fwd[i].spec - arena;
+    if (off < 0 || off % elems) return false;
+    slot.push_back((int)(off / elems));
+    items.push_back(&fwd[i]);
+  }
+  int max_slot = 0;
+  for (int sl : slot) max_slot = std::max(max_slot, sl);
+  if ((int64_t)(max_slot + 1) * qz > 65535 || (int64_t)fwd.size() * qz > ztab_entries(qz)) return false;
+  ztab->assign(fwd.size() * qz, 0);
+  tasks->clear();
+  for (const PlaneGroup& g : plane_groups(items, g_plane_dedup)) {
+    const std::pair<int, int>* keep = nullptr;
+    for (const auto& o : g.occ)
+      if (o.second < zp[o.first].zlo || o.second >= zp[o.first].zhi) {
+        keep = &o;
+        break;
+      }
+    if (keep) {
+      tasks->push_back(make_int2(keep->first, keep->second));
+      for (const auto& o : g.occ) (*ztab)[o.first * qz + o.second] = (unsigned short)(slot[keep->first] * qz + keep->second);
+    } else {
+      for (const auto& o : g.occ) {
+        tasks->push_back(make_int2(o.first, o.second));
+        (*ztab)[o.first * qz + o.second] = (unsigned short)(slot[o.first] * qz + o.second);
+      }
+    }
+  }
+  // the planes of one item and position next to each other: neighbouring
+  // workgroups read neighbouring source rows
+  std::stable_sort(tasks->begin(), tasks->end(), [](int2 a, int2 b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
   return true;
 }
 }  // namespace mvd
@@ -686,7 +734,8 @@ int timing_class(int kind, int path, int epi) {
 // fwd_tasks: the task list of the fused forward group when the caller planned
 // it (plan_conv_planes); otherwise forward_plane_tasks makes one.
 int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaunch>* out,
-                        const std::vector<PlaneTask>* fwd_tasks = nullptr) {
+                        const std::vector<PlaneTask>* fwd_tasks = nullptr,
+                        const std::vector<int2>* plane_tasks = nullptr, bool wide = false) {
   // z items (register kernels) that share a kernel spectrum and shape with at
   // least one other item form their own group: one "lines" launch reads the
   // kernel spectrum once for all of them (the blocks of one view of one stack)
@@ -745,7 +794,7 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       int with_table = 0;
       for (const PassDesc* d : kv.second) with_table += d->ztab != nullptr;
       MVD_REQUIRE(with_table == 0 || with_table == L.nitems, "internal: z items with and without plane tables");
-      L.table = with_table > 0;
+      L.table = with_table > 0 ? (wide ? 2 : 1) : 0;
       // the lines kernel loads an item's lines while it stores the previous
       // item's: the spectra must be distinct buffers
       std::set<const float2*> specs;
@@ -771,6 +820,22 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       L.nwg = (int)tasks.size();
       L.has_sched = true;
     }
+    if (plane_tasks && L.path == 1 && (L.kind == kPassXForward || L.kind == kPassYForward)) {
+      // every item of the pass in this group, in the order of descs
+      MVD_REQUIRE(kv.second.size() == descs.size(), "internal: plane tasks over several launch groups");
+      const PassDesc& d0 = *kv.second.front();
+      if (L.kind == kPassXForward) {
+        const int rows = x_fast_rows_per_wg(d0.g.q.x / 2);
+        MVD_REQUIRE(rows > 0 && d0.g.q.y % rows == 0, "internal: rows per plane");
+        L.wpt = d0.g.q.y / rows;
+      } else {
+        L.wpt = gxs.front();
+      }
+      for (const PassDesc* d : kv.second) MVD_REQUIRE(d->g.q.y == d0.g.q.y && d->zlo == 0, "internal: plane task items");
+      MVD_CHECK(L.ptasks.ensure(sizeof(int2) * plane_tasks->size()));
+      MVD_CHECK(hipMemcpy(L.ptasks.p, plane_tasks->data(), sizeof(int2) * plane_tasks->size(), hipMemcpyHostToDevice));
+      L.nwg = (int)plane_tasks->size() * L.wpt;
+    }
     out->push_back(std::move(L));
   }
   return 0;
@@ -783,7 +848,8 @@ hipError_t enqueue_pass_launch(const PassLaunch& L, hipStream_t st) {
     return launch_batch_xy(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st,
                            L.has_sched ? L.sched.as<PlaneTask>() : nullptr);
   if (L.path == 1)
-    return launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st);
+    return launch_batch_fast(L.kind, L.cls, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, st,
+                             L.wpt ? L.ptasks.as<int2>() : nullptr, L.wpt);
   return launch_batch_generic(L.kind, L.epi, L.items.p, L.prefix.as<int>(), L.gx.as<int>(), L.nitems, L.nwg, L.lds,
                               st);
 }
@@ -881,19 +947,26 @@ int build_block_batch(mvd_session* s) {
   MVD_CHECK(hipMemsetAsync(bb.arena.p, 0, need, s->stream));  // finite row padding
   std::vector<float2*> spec;
   for (size_t b = 0; b < nb; ++b) spec.push_back(bb.arena.as<float2>() + b * elems);
-  // the lines kernel addresses the arena with 32-bit offsets
-  const bool tables = g_xcd_schedule && need < (size_t(1) << 31);
+  // plane tables: the fused planes' lines kernel addresses the arena with
+  // 32-bit offsets; the separate passes' one with 64-bit ones past 2^31 bytes
+  const bool wide = need >= (size_t(1) << 31);
+  const bool tables = g_xcd_schedule && !wide;
   bb.view.resize(s->nviews);
   for (int v = 0; v < s->nviews; ++v)
     for (int conv = 0; conv < 2; ++conv) {
       std::vector<PassDesc> pass[5];
       conv_pass_descs(s, v, conv, spec, pass);
       std::vector<PlaneTask> tasks;
+      std::vector<int2> ptasks;
       std::vector<unsigned short> tab;
       const bool planned = tables && pass[0].size() == pass[2].size() && !pass[0].empty() && pass[0][0].xy &&
                            pass_is_fast(pass[2][0]) &&
                            plan_conv_planes(pass[0], pass[2], bb.arena.as<float2>(), (int64_t)elems, &tasks, &tab);
-      if (planned) {
+      const bool sep = !planned && g_xcd_schedule && pass[0].size() == pass[2].size() && !pass[0].empty() &&
+                       !pass[0][0].xy && pass[1].size() == pass[0].size() && pass_is_fast(pass[0][0]) &&
+                       pass_is_fast(pass[1][0]) && pass_is_fast(pass[2][0]) &&
+                       plan_sep_planes(pass[0], pass[2], bb.arena.as<float2>(), (int64_t)elems, &ptasks, &tab);
+      if (planned || sep) {
         bb.ztab.emplace_back();
         DeviceBuf& dt = bb.ztab.back();
         MVD_CHECK(dt.ensure(sizeof(unsigned short) * tab.size()));
@@ -902,11 +975,12 @@ int build_block_batch(mvd_session* s) {
         for (size_t i = 0; i < pass[2].size(); ++i) {
           pass[2][i].ztab = dt.as<unsigned short>() + i * qz;
           pass[2][i].arena = bb.arena.as<float2>();
-          pass[2][i].arena_bytes = (unsigned)need;
+          pass[2][i].arena_bytes = wide ? 0u : (unsigned)need;
         }
       }
       for (int p = 0; p < 5; ++p) {
-        int rc = build_pass_launches(pass[p], &bb.view[v], p == 0 && planned ? &tasks : nullptr);
+        int rc = build_pass_launches(pass[p], &bb.view[v], p == 0 && planned ? &tasks : nullptr,
+                                     sep && p < 2 ? &ptasks : nullptr, wide);
         if (rc) return rc;
       }
     }

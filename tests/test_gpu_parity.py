@@ -182,6 +182,26 @@ def test_deconvolution_matches_oracle(ptype, lam, block):
     d.close()
 
 
+@pytest.mark.parametrize("size,block", [((100, 90, 80), (64, 64, 64)), ((150, 140, 70), (128, 128, 64))])
+def test_block_batch_separate_passes_matches_oracle(size, block):
+    """Several blocks whose windows take the separate register x and y passes
+    (not the fused 256 x 256 planes): the batched iteration transforms each
+    distinct window plane once (windows that cross an image face in z mirror
+    planes another window holds) and the z pass reads the planes through
+    tables; the result must equal the oracle's block by block convolution."""
+    ds = S.make_dataset(size, 2, [0, 90], 40, (7, 7, 9), 1.2, 2.5, seed=sum(size))
+    d = _session(ds.views, ds.psfs, PSFTYPE.EFFICIENT_BAYESIAN, 3, 0.006, block=block)
+    g = d.geometry(0)
+    print("separate passes", size, block, g, flush=True)
+    assert g["blocks"] > 1
+    psi = d.get_psi()
+    ref, st = O.deconvolve(ds.views, ds.psfs, O.EFFICIENT_BAYESIAN, 3, 0.006, block_xyz=block, norm_threads=8)
+    assert rel_max(psi, ref) < GATE
+    gst = np.array(d.stats)
+    np.testing.assert_allclose(gst[:, :, 0], st[:, :, 0], rtol=1e-3)
+    d.close()
+
+
 @pytest.mark.parametrize("wtype,osem_index", [(WeightType.AS_GIVEN, 0), (WeightType.PRECOMPUTED_WEIGHTS, 1),
                                               (WeightType.VIRTUAL_WEIGHTS, 2), (WeightType.PRECOMPUTED_WEIGHTS, 3)])
 def test_weighted_deconvolution_matches_oracle(wtype, osem_index):
