@@ -133,7 +133,8 @@ def run_c3(args):
     """Config 3: 32 independent stacks, 15 iterations.  Plain = each stack
     alone (mvd_iterate, default FFT shapes); batched = all stacks in one
     batch (one launch per pass and kernel class, hipGraph replay, autotuned
-    FFT shapes).  Also reports batching alone (same shapes, graph)."""
+    FFT shapes).  Also reports batching alone, with the default shapes and
+    with the autotuned shapes on both sides."""
     import torch
     from spim_registration_amd import FFTPolicy, MVDeconBatch, MVDeconFFT, MVDeconInput, MVDeconvolution, PSFTYPE
     from spim_registration_amd import synthetic as S
@@ -154,18 +155,19 @@ def run_c3(args):
         return out
 
     res = {}
-    # plain per-stack path
-    plain = sessions(FFTPolicy.SMOOTH)
-    for d in plain:
-        d.iterate(1)  # warm-up (kernel loading)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for d in plain:
-        d.iterate(iters)
-    torch.cuda.synchronize()
-    res["plain_s"] = time.perf_counter() - t0
-    for d in plain:
-        d.close()
+    # plain per-stack path, default and autotuned shapes
+    for name, policy in (("plain_s", FFTPolicy.SMOOTH), ("plain_autotuned_s", FFTPolicy.AUTOTUNE)):
+        plain = sessions(policy)
+        for d in plain:
+            d.iterate(1)  # warm-up (kernel loading)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for d in plain:
+            d.iterate(iters)
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+        for d in plain:
+            d.close()
     for name, policy in (("batched_same_shapes_s", FFTPolicy.SMOOTH), ("batched_s", FFTPolicy.AUTOTUNE)):
         t = time.perf_counter()
         st = sessions(policy)
@@ -192,7 +194,9 @@ def run_c3(args):
         "plain_gunits_per_s": round(units / res["plain_s"] / 1e9, 3),
         "batched_same_shapes_gunits_per_s": round(units / res["batched_same_shapes_s"] / 1e9, 3),
         "batched_gunits_per_s": round(units / res["batched_s"] / 1e9, 3),
+        "plain_autotuned_gunits_per_s": round(units / res["plain_autotuned_s"] / 1e9, 3),
         "speedup_batching_only": round(res["plain_s"] / res["batched_same_shapes_s"], 3),
+        "speedup_batching_only_autotuned": round(res["plain_autotuned_s"] / res["batched_s"], 3),
         "seconds": {k: round(v, 4) for k, v in res.items()},
     }
     print(json.dumps(out), flush=True)
