@@ -11,6 +11,54 @@
 
 namespace mvd {
 
+// Workgroup order of the y and z line kernels.  A workgroup holding L lines
+// of 8 bytes covers L * 8 bytes of every row it touches; at 64 bytes (the
+// 512-point lengths) two workgroups share each 128-byte line.  Those kernels,
+// and the batched z kernel at every length, run their workgroups in the
+// order of xcd_chunk: the two halves of a line are read by one XCD (one HBM
+// fetch instead of two) and every XCD's accesses spread over its L2 channels.
+
+// lines per workgroup of the kernels of length N
+template <int N>
+static constexpr int lines_per_wg() {
+  return Spec<N>::NT / (N / Spec<N>::E);
+}
+
+template <int L, bool kAlways>
+constexpr bool chunked_order() {
+  return kAlways || L * sizeof(float2) < 128;
+}
+
+// (column group, row) of a 2-D grid, column groups fastest
+template <int L, bool kAlways = false>
+__device__ __forceinline__ void line_group(int& bx, int& by) {
+  if constexpr (chunked_order<L, kAlways>()) {
+    const int w = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    bx = w % gridDim.x;
+    by = w / gridDim.x;
+  } else {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+  }
+}
+
+// logical workgroup of a 1-D (batched) grid
+template <int L>
+__device__ __forceinline__ int line_wg() {
+  if constexpr (chunked_order<L, false>()) return xcd_chunk(blockIdx.x, gridDim.x);
+  return blockIdx.x;
+}
+
+// Column groups of a y pass grid: with 64-byte groups they cover the row
+// pitch, so that the padding lanes complete the row's last 128-byte line with
+// zeros.  (The z passes cover hx: an extra group per row costs them more than
+// the partial line.)
+static int y_line_groups(const ShapePlan& sp, int L) {
+  return L * (int)sizeof(float2) < 128 ? sp.pitch / L : (sp.hx + L - 1) / L;
+}
+
+static int z_line_groups(const ShapePlan& sp, int L) { return (sp.hx + L - 1) / L; }
+
 // ---------------------------------------------------------------------------
 // y pass: complex FFT along y of L consecutive kx columns for one z.
 
@@ -64,7 +112,9 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
 
 template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_y_fast(YFastParams P) {
-  y_fast_body<N>(P, blockIdx.x, blockIdx.y);
+  int bx, by;
+  line_group<lines_per_wg<N>()>(bx, by);
+  y_fast_body<N>(P, bx, by);
 }
 
 // tasks (optional): one (item, plane) per wpt workgroups -- only the planes
@@ -74,13 +124,14 @@ template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_y_fast_batch(const YFastParams* items, const int* prefix, const int* gx,
                                                    int nitems, const int2* tasks, int wpt) {
   int item, bx, by;
+  const int wg = line_wg<lines_per_wg<N>()>();
   if (tasks) {
-    const int2 t = tasks[blockIdx.x / wpt];
+    const int2 t = tasks[wg / wpt];
     item = t.x;
-    bx = blockIdx.x % wpt;
+    bx = wg % wpt;
     by = t.y - items[item].zlo;
   } else {
-    batch_locate(prefix, gx, nitems, item, bx, by);
+    batch_locate(prefix, gx, nitems, item, bx, by, wg);
   }
   const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
   y_fast_body<N>(item_params, bx, by);
@@ -229,7 +280,9 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
     }
     __syncthreads();
   }
-  const int col = blockIdx.x * L + line;
+  int bx, by;
+  line_group<L, true>(bx, by);
+  const int col = bx * L + line;
   const uint32_t plane = (uint32_t)(P0.qy * P0.pitch) * 8u;  // bytes
   const uint32_t bytes = plane * (uint32_t)N;
   // Lanes of the padding columns [hx, pitch) transform their own (unused)
@@ -237,14 +290,14 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
   // (a partly written line reaches HBM as a read-modify-write).  Lanes past the
   // pitch read column hx - 1 and store nothing.  A line's transform never
   // mixes with other lines.
-  const uint32_t row = (uint32_t)(blockIdx.y * P0.pitch) * 8u;  // uniform: the SGPR offset
+  const uint32_t row = (uint32_t)(by * P0.pitch) * 8u;  // uniform: the SGPR offset
   const bool store_col = col < P0.pitch;
   const uint32_t off = (uint32_t)(store_col ? col : P0.hx - 1) * 8u;
   auto addr = [line](int i) { return i * LP + line; };
   auto load_line = [&](int it, const float2* spec, float2(&v)[E]) {
     if constexpr (TAB == 2) {
       // 64-bit: arena + plane index * plane + row + column
-      const auto base = gmem(P0.arena) + (size_t)(blockIdx.y * P0.pitch) + (off >> 3);
+      const auto base = gmem(P0.arena) + (size_t)(by * P0.pitch) + (off >> 3);
       const size_t plane_elems = (size_t)P0.qy * P0.pitch;
 #pragma unroll
       for (int b = 0; b < E / S::R0; ++b)
@@ -336,7 +389,9 @@ hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, 
 
 template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast(ZFastParams P) {
-  z_fast_body<N>(P, blockIdx.x, blockIdx.y);
+  int bx, by;
+  line_group<lines_per_wg<N>()>(bx, by);
+  z_fast_body<N>(P, bx, by);
 }
 
 // z pass of the likelihood gradient on registers (see k_z_grad in
@@ -370,8 +425,9 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
   __shared__ float2 lds[N * LP];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
-  const int col = blockIdx.x * L + line;
-  const int ky = blockIdx.y;
+  int bx, ky;
+  line_group<L>(bx, ky);
+  const int col = bx * L + line;
   const bool valid = col < P.hx;
   // spectra hold fewer than 2^31 elements (lengths up to 1024): 32-bit offsets
   const int plane = P.q.y * P.pitch;
@@ -467,7 +523,7 @@ hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* w
 #define MVD_ZG(N)                                                                               \
   case N: {                                                                                     \
     constexpr int L = Spec<N>::NT / (N / Spec<N>::E);                                           \
-    dim3 grid((sp.hx + L - 1) / L, sp.q.y);                                                     \
+    dim3 grid(z_line_groups(sp, L), sp.q.y);                                                    \
     hipLaunchKernelGGL(k_z_fast_grad<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                \
     break;                                                                                      \
   }
@@ -488,7 +544,7 @@ template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_batch(const ZFastParams* items, const int* prefix, const int* gx,
                                                    int nitems) {
   int item, bx, by;
-  batch_locate(prefix, gx, nitems, item, bx, by);
+  batch_locate(prefix, gx, nitems, item, bx, by, line_wg<lines_per_wg<N>()>());
   const auto item_params = items[item];  // private copy: the kernel's stores cannot alias it
   z_fast_body<N>(item_params, bx, by);
 }
@@ -930,11 +986,6 @@ __global__ __launch_bounds__(Spec<M>::NT) void k_x_inverse_fast_batch(const XInv
 // ---------------------------------------------------------------------------
 // dispatch
 
-template <int N>
-static constexpr int lines_per_wg() {
-  return Spec<N>::NT / (N / Spec<N>::E);
-}
-
 // MVD_FAST_SMOOTH=0 keeps the 144- and 288-point lengths on the generic LDS
 // kernels (comparisons).
 static const bool g_fast_smooth = [] {
@@ -1011,8 +1062,7 @@ int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) 
     case kPassYForward:
     case kPassYInverse: {
       YFastParams P{d.spec, sp.tw_y, sp.hx, sp.pitch, d.zlo, d.ylo, d.yhi, d.dir};
-      const int L = fast_lines(sp.q.y);
-      *gx = (sp.hx + L - 1) / L;
+      *gx = y_line_groups(sp, fast_lines(sp.q.y));
       *gy = d.zhi - d.zlo;
       *cls = sp.q.y;
       std::memcpy(out, &P, sizeof(P));
@@ -1021,8 +1071,7 @@ int build_fast_params(const PassDesc& d, void* out, int* gx, int* gy, int* cls) 
     case kPassZConv: {
       ZFastParams P{d.spec, d.kspec, sp.tw_z, d.scale, d.dir, sp.hx, sp.pitch, sp.q.y, d.zlo, d.zhi,
                     d.ztab, d.arena, d.arena_bytes};
-      const int L = fast_lines(sp.q.z);
-      *gx = (sp.hx + L - 1) / L;
+      *gx = z_line_groups(sp, fast_lines(sp.q.z));
       *gy = sp.q.y;
       *cls = sp.q.z;
       std::memcpy(out, &P, sizeof(P));
@@ -1131,7 +1180,7 @@ hipError_t launch_y_fast_window(const ShapePlan& sp, float2* spec, int zlo, int 
   (void)hipGetLastError();
 #define MVD_Y(N)                                                                                \
   case N: {                                                                                     \
-    dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), zhi - zlo);                  \
+    dim3 grid(y_line_groups(sp, lines_per_wg<N>()), zhi - zlo);                                 \
     hipLaunchKernelGGL(k_y_fast<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                     \
     break;                                                                                      \
   }
@@ -1159,7 +1208,7 @@ hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec,
   (void)hipGetLastError();
 #define MVD_Z(N)                                                                                \
   case N: {                                                                                     \
-    dim3 grid((sp.hx + lines_per_wg<N>() - 1) / lines_per_wg<N>(), sp.q.y);                     \
+    dim3 grid(z_line_groups(sp, lines_per_wg<N>()), sp.q.y);                                    \
     hipLaunchKernelGGL(k_z_fast<N>, grid, dim3(Spec<N>::NT), 0, stream, P);                     \
     break;                                                                                      \
   }

@@ -67,12 +67,27 @@ struct ShapePlan {
   int64_t spectrum_elems() const { return (int64_t)pitch * q.y * q.z; }
 };
 
-// Batched launches: workgroup blockIdx.x belongs to the item i with
-// prefix[i] <= blockIdx.x < prefix[i + 1]; within the item it is workgroup
-// (local % gx[i], local / gx[i]) of that item's own grid.
+// The dispatcher hands a launch's workgroups to the 8 XCDs in turn (workgroup
+// w to XCD w mod 8) and each XCD has its own L2.  xcd_chunk maps the running
+// workgroup w to the logical one it works on so that each XCD takes one
+// contiguous eighth of the logical range, in order: logical neighbours run on
+// one XCD at about the same time and share its L2 (lines that two workgroups
+// each read half of are fetched once), and an XCD's accesses spread over all
+// of its L2 channels instead of the few that a fixed stride of 8 workgroups
+// selects.  The tail past the last whole round keeps its order.  A
+// permutation of [0, count) for every count.
+__device__ __forceinline__ int xcd_chunk(int w, int count) {
+  const int per = count >> 3;
+  if (w >= per * 8) return w;
+  return (w & 7) * per + (w >> 3);
+}
+
+// Batched launches: workgroup wg (blockIdx.x, or its place in a reordered
+// launch) belongs to the item i with prefix[i] <= wg < prefix[i + 1]; within
+// the item it is workgroup (local % gx[i], local / gx[i]) of that item's own
+// grid.
 __device__ __forceinline__ void batch_locate(const int* prefix, const int* gx, int nitems, int& item, int& bx,
-                                             int& by) {
-  const int wg = blockIdx.x;
+                                             int& by, int wg = blockIdx.x) {
   int lo = 0, hi = nitems;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;

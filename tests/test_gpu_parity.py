@@ -43,7 +43,7 @@ def test_device_query(jna):
     # power-of-two FFT lengths take the register-resident kernels
     ((64, 64, 64), (13, 11, 9)), ((128, 128, 128), (31, 19, 19)), ((64, 256, 256), (51, 31, 31)),
     ((64, 64, 512), (9, 9, 41)), ((32, 64, 1024), (5, 7, 9)), ((512, 64, 64), (81, 9, 9)),
-    ((256, 128, 128), (3, 3, 3)),
+    ((256, 128, 128), (3, 3, 3)), ((64, 512, 48), (9, 31, 9)),
     # 288- and 144-point lengths (288 = 24 x 12, 144 = 12 x 12) take the
     # register-resident kernels too; 100 (x) the generic ones
     ((288, 144, 144), (31, 19, 19)), ((100, 288, 280), (9, 31, 51)),
@@ -182,7 +182,9 @@ def test_deconvolution_matches_oracle(ptype, lam, block):
     d.close()
 
 
-@pytest.mark.parametrize("size,block", [((100, 90, 80), (64, 64, 64)), ((150, 140, 70), (128, 128, 64))])
+@pytest.mark.parametrize("size,block", [((100, 90, 80), (64, 64, 64)), ((150, 140, 70), (128, 128, 64)),
+                                        # 512-point y and z lines: two workgroups per 128-byte row line
+                                        ((130, 540, 40), (128, 512, 32)), ((130, 40, 540), (128, 32, 512))])
 def test_block_batch_separate_passes_matches_oracle(size, block):
     """Several blocks whose windows take the separate register x and y passes
     (not the fused 256 x 256 planes): the batched iteration transforms each
