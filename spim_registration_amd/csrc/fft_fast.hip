@@ -74,10 +74,10 @@ struct YFastParams {
 template <int N>
 __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
-  __shared__ float2 lds[N * LP];
+  __shared__ float2 lds[line_slots<N, L>()];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
   // lanes of the row padding [hx, pitch) transform zeros and store them, so
@@ -96,7 +96,7 @@ __device__ __forceinline__ void y_fast_body(const YFastParams& P, int bx, int by
       const bool in = valid && plane_in && y >= P.iylo && y < P.iyhi;
       v[b * S::R0 + t] = in ? maybe_stream_load<kStream>(&base[y * P.pitch]) : make_float2(0.f, 0.f);
     }
-  auto addr = [line](int i) { return i * LP + line; };
+  auto addr = [line](int i) { return line_slot<N, L>(i, line); };
   fft_forward_order<N>(v, lds, jt, tw, 1, P.dir, addr);
   constexpr int RL = last_radix<N>();
   if (store_col) {
@@ -164,10 +164,10 @@ struct ZFastParams {
 template <int N>
 __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   __shared__ float2 tw[N];
-  __shared__ float2 lds[N * LP];
+  __shared__ float2 lds[line_slots<N, L>()];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
   const int col = bx * L + line;
@@ -198,7 +198,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
         if (valid) v[b * S::R0 + t] = maybe_stream_load<kStream>(&gmem(P.spec)[z * plane + off]);
       }
   }
-  auto addr = [line](int i) { return i * LP + line; };
+  auto addr = [line](int i) { return line_slot<N, L>(i, line); };
   // the kernel spectrum is loaded with the line, in the order the forward
   // transform leaves its outputs, so its latency hides behind the transform
   float2 kv[E];
@@ -263,11 +263,11 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 template <int N, int TAB>
 __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
-  __shared__ float2 lds[N * LP];
+  __shared__ float2 lds[line_slots<N, L>()];
   constexpr bool TABLE = TAB != 0;
   __shared__ unsigned short ztab[TABLE ? ztab_entries(N) : 1];
   const ZFastParams P0 = items[0];
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z
   const uint32_t row = (uint32_t)(by * P0.pitch) * 8u;  // uniform: the SGPR offset
   const bool store_col = col < P0.pitch;
   const uint32_t off = (uint32_t)(store_col ? col : P0.hx - 1) * 8u;
-  auto addr = [line](int i) { return i * LP + line; };
+  auto addr = [line](int i) { return line_slot<N, L>(i, line); };
   auto load_line = [&](int it, const float2* spec, float2(&v)[E]) {
     if constexpr (TAB == 2) {
       // 64-bit: arena + plane index * plane + row + column
@@ -419,10 +419,10 @@ struct ZGradFastParams {
 template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) {
   using S = Spec<N>;
-  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T, LP = L + 1;
+  constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr int RL = last_radix<N>();
   __shared__ float2 tw[N];
-  __shared__ float2 lds[N * LP];
+  __shared__ float2 lds[line_slots<N, L>()];
   const int line = threadIdx.x % L, jt = threadIdx.x / L;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = gmem(P.tw)[i];
   int bx, ky;
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
       v[b * S::R0 + t] = make_float2(0.f, 0.f);
       if (valid) v[b * S::R0 + t] = gmem(P.spec)[z * plane + off];
     }
-  auto addr = [line](int i) { return i * LP + line; };
+  auto addr = [line](int i) { return line_slot<N, L>(i, line); };
   fft_forward_order<N>(v, lds, jt, tw, 1, -1.f, addr);
   // R^(0) (kz = 0: v[0] of thread jt = 0 of line (kx, ky) = (0, 0)) from the double sum
   if (P.rsum && ky == 0 && col == 0 && jt == 0) v[0] = make_float2((float)*gmem(P.rsum), 0.f);

@@ -147,6 +147,30 @@ struct Spec<512> {
   static constexpr int E = 16, R0 = 8, R1 = 8, R2 = 8, NT = kThreads;
 };
 
+// Slot of element i of line `line` in the line-interleaved exchange buffer of
+// the y and z kernels (L lines per workgroup, line_slots<N, L>() entries).
+// LDS serves a 64-bit access 32 lanes at a time from 64 four-byte banks; a
+// half wave holds the L lines of 32 / L thread groups jt, and the groups must
+// land in disjoint banks.  With rows of L + 1 slots the first pass's stores
+// (elements R0 jt + t) are disjoint, but the loads (elements jt + T b +
+// t N / R) of two groups share a bank pair (256 points; more at 512), so every
+// such access takes an extra cycle.  For 256 and 512 points the rows are L
+// slots wide and offset by i & 16 (256) or i & 24 (512) slots, which puts the
+// groups of a half wave in disjoint banks in every exchange of both transform
+// orders, plus 32 slots per 32 elements so that rows do not overlap.  Other
+// lengths keep rows of L + 1 slots.
+template <int N, int L>
+__device__ __forceinline__ int line_slot(int i, int line) {
+  if constexpr (N == 256 && L == 16) return (i << 4) + line + (i & 16) + (i & ~31);
+  else if constexpr (N == 512 && L == 8) return (i << 3) + line + (i & 24) + (i & ~31);
+  else return i * (L + 1) + line;
+}
+
+template <int N, int L>
+constexpr int line_slots() {
+  return N * (L + 1);
+}
+
 // Butterflies of one pass on the registers (inputs already in pass order).
 template <int N, int E, int R, int NS>
 __device__ __forceinline__ void bfly_pass(float2 (&v)[E], int jt, const float2* tw, int tw_step, float dir) {
