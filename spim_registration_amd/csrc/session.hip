@@ -965,6 +965,7 @@ int build_block_batch(mvd_session* s) {
       const bool sep = !planned && g_xcd_schedule && pass[0].size() == pass[2].size() && !pass[0].empty() &&
                        !pass[0][0].xy && pass[1].size() == pass[0].size() && pass_is_fast(pass[0][0]) &&
                        pass_is_fast(pass[1][0]) && pass_is_fast(pass[2][0]) &&
+                       pass[0][0].g.q.y % x_fast_rows_per_wg(pass[0][0].g.q.x / 2) == 0 &&
                        plan_sep_planes(pass[0], pass[2], bb.arena.as<float2>(), (int64_t)elems, &ptasks, &tab);
       if (planned || sep) {
         bb.ztab.emplace_back();

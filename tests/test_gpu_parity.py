@@ -184,7 +184,10 @@ def test_deconvolution_matches_oracle(ptype, lam, block):
 
 @pytest.mark.parametrize("size,block", [((100, 90, 80), (64, 64, 64)), ((150, 140, 70), (128, 128, 64)),
                                         # 512-point y and z lines: two workgroups per 128-byte row line
-                                        ((130, 540, 40), (128, 512, 32)), ((130, 40, 540), (128, 32, 512))])
+                                        ((130, 540, 40), (128, 512, 32)), ((130, 40, 540), (128, 32, 512)),
+                                        # 144 rows per plane are not a multiple of the 64-point x
+                                        # kernel's 32 rows per workgroup: no per-plane x tasks
+                                        ((150, 160, 80), (128, 144, 64))])
 def test_block_batch_separate_passes_matches_oracle(size, block):
     """Several blocks whose windows take the separate register x and y passes
     (not the fused 256 x 256 planes): the batched iteration transforms each
