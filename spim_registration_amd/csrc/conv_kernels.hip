@@ -371,11 +371,11 @@ __device__ __forceinline__ void x_inverse_body(const XInvParams& P, int bx, int 
     if (e.mode == kEpiStore) {
       gmem(e.dst)[gi] = v;
     } else if (e.mode == kEpiResidual) {
-      const double w = e.weight_mode == kWeightArray ? (double)gmem(e.weight)[gi] : 1.0;
-      const double b = (double)v, y = (double)gmem(e.img)[gi];
-      if (b > 0) {
-        my_sum += w * (b - y * log(b));
-        gmem(e.dst)[gi] = (float)(w * (1.0 - y / b));
+      const float w = e.weight_mode == kWeightArray ? gmem(e.weight)[gi] : 1.f;
+      const float y = gmem(e.img)[gi];
+      if (v > 0.f) {
+        my_sum += nll_term(w, v, y);
+        gmem(e.dst)[gi] = nll_residual(w, v, y);
       } else {
         gmem(e.dst)[gi] = 0.f;
       }

@@ -891,11 +891,11 @@ __device__ __forceinline__ void x_inverse_fast_body(const XInvFastParams& P, int
         } else if (EPI == kEpiQuotient) {
           gmem(e.dst)[gi] = rl_quotient(val, ch.in0[rr][ee]);
         } else if (EPI == kEpiResidual) {
-          const double w = (double)ch.in1[rr][ee], b = (double)val, y = (double)ch.in0[rr][ee];
+          const float w = ch.in1[rr][ee], y = ch.in0[rr][ee];
           float rv = 0.f;
-          if (b > 0) {
-            my_sum += w * (b - y * log(b));
-            rv = (float)(w * (1.0 - y / b));
+          if (val > 0.f) {
+            my_sum += nll_term(w, val, y);
+            rv = nll_residual(w, val, y);
           }
           if (r_rows) {
             fbuf[l * 2 * MP + g.eff_local.x + xo] = rv;  // the value this thread read

@@ -447,10 +447,9 @@ __device__ __forceinline__ void xy_inverse_body(const XYInvParams& P, int bx) {
       } else if (EPI == kEpiQuotient) {
         gmem(e.dst)[gi] = rl_quotient(val, in0[i]);
       } else if (EPI == kEpiResidual) {
-        const double w = (double)in1[i], b = (double)val, yv = (double)in0[i];
-        if (b > 0) {
-          my_sum += w * (b - yv * log(b));
-          gmem(e.dst)[gi] = (float)(w * (1.0 - yv / b));
+        if (val > 0.f) {
+          my_sum += nll_term(in1[i], val, in0[i]);
+          gmem(e.dst)[gi] = nll_residual(in1[i], val, in0[i]);
         } else {
           gmem(e.dst)[gi] = 0.f;
         }
