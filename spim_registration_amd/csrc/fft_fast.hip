@@ -1,4 +1,4 @@
-// Register-resident FFT passes for lengths 64, 128, 144, 256, 288 and 512:
+// Register-resident FFT passes for lengths 64, 128, 144, 192, 256, 288, 384 and 512:
 // the y pass, the fused z pass (forward, kernel product, inverse) and the
 // real-to-complex / complex-to-real x passes with their gather and epilogue.
 // The butterfly and LDS exchange building blocks live in fft_regs.h.
@@ -261,7 +261,7 @@ __device__ __forceinline__ void z_fast_body(const ZFastParams& P, int bx, int by
 // with 32-bit buffer offsets (fewer than 2^31 bytes), TAB = 2 with 64-bit
 // addresses.
 template <int N, int TAB>
-__global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
+__global__ __launch_bounds__(Spec<N>::NT, N == 512 || N == 288 || N == 192 || N == 384 ? 2 : 3) void k_z_fast_lines(const ZFastParams* __restrict__ items, int nitems) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr bool kStream = L * sizeof(float2) % 128 == 0;  // whole lines per workgroup
@@ -379,7 +379,13 @@ hipError_t launch_z_lines(int n, const void* items, int nitems, int gx, int gy, 
     case 128: MVD_Z_LINES(128) break;
     case 144: MVD_Z_LINES(144) break;
     case 256: MVD_Z_LINES(256) break;
+    case 192: MVD_Z_LINES(192) break;
     case 288: MVD_Z_LINES(288) break;
+    case 384:  // plane tables only (see the launch grouping)
+      if (table == 2) hipLaunchKernelGGL((k_z_fast_lines<384, 2>), grid, dim3(Spec<384>::NT), 0, stream, it, nitems);
+      else if (table == 1) hipLaunchKernelGGL((k_z_fast_lines<384, 1>), grid, dim3(Spec<384>::NT), 0, stream, it, nitems);
+      else return hipErrorInvalidValue;
+      break;
     case 512: MVD_Z_LINES(512) break;
     default: return hipErrorInvalidValue;
   }
@@ -532,7 +538,9 @@ hipError_t launch_z_fast_grad(const ShapePlan& sp, float2* spec, const float2* w
     MVD_ZG(128)
     MVD_ZG(144)
     MVD_ZG(256)
+    MVD_ZG(192)
     MVD_ZG(288)
+    MVD_ZG(384)
     MVD_ZG(512)
     default: return hipErrorInvalidValue;
   }
@@ -986,15 +994,16 @@ __global__ __launch_bounds__(Spec<M>::NT) void k_x_inverse_fast_batch(const XInv
 // ---------------------------------------------------------------------------
 // dispatch
 
-// MVD_FAST_SMOOTH=0 keeps the 144- and 288-point lengths on the generic LDS
-// kernels (comparisons).
+// MVD_FAST_SMOOTH=0 keeps the 144-, 192-, 288- and 384-point lengths on the
+// generic LDS kernels (comparisons).
 static const bool g_fast_smooth = [] {
   const char* v = std::getenv("MVD_FAST_SMOOTH");
   return !(v && v[0] == '0');
 }();
 
 bool fast_size(int n) {
-  return n == 64 || n == 128 || n == 256 || n == 512 || (g_fast_smooth && (n == 144 || n == 288));
+  return n == 64 || n == 128 || n == 256 || n == 512 ||
+         (g_fast_smooth && (n == 144 || n == 192 || n == 288 || n == 384));
 }
 
 int x_fast_rows_per_wg(int M) {
@@ -1003,7 +1012,9 @@ int x_fast_rows_per_wg(int M) {
     case 128: return lines_per_wg<128>();
     case 144: return lines_per_wg<144>();
     case 256: return lines_per_wg<256>();
+    case 192: return lines_per_wg<192>();
     case 288: return lines_per_wg<288>();
+    case 384: return lines_per_wg<384>();
     case 512: return lines_per_wg<512>();
   }
   return 0;
@@ -1040,7 +1051,9 @@ static int fast_lines(int n) {
     case 128: return lines_of<128>();
     case 144: return lines_of<144>();
     case 256: return lines_of<256>();
+    case 192: return lines_of<192>();
     case 288: return lines_of<288>();
+    case 384: return lines_of<384>();
     case 512: return lines_of<512>();
   }
   return 1;
@@ -1108,7 +1121,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
         case 128: hipLaunchKernelGGL(k_y_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         case 144: hipLaunchKernelGGL(k_y_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         case 256: hipLaunchKernelGGL(k_y_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 192: hipLaunchKernelGGL(k_y_fast_batch<192>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         case 288: hipLaunchKernelGGL(k_y_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
+        case 384: hipLaunchKernelGGL(k_y_fast_batch<384>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         case 512: hipLaunchKernelGGL(k_y_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems, tasks, wpt); break;
         default: return hipErrorInvalidValue;
       }
@@ -1121,7 +1136,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
         case 128: hipLaunchKernelGGL(k_z_fast_batch<128>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 144: hipLaunchKernelGGL(k_z_fast_batch<144>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 256: hipLaunchKernelGGL(k_z_fast_batch<256>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 192: hipLaunchKernelGGL(k_z_fast_batch<192>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 288: hipLaunchKernelGGL(k_z_fast_batch<288>, grid, block, 0, stream, it, prefix, gx, nitems); break;
+        case 384: hipLaunchKernelGGL(k_z_fast_batch<384>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         case 512: hipLaunchKernelGGL(k_z_fast_batch<512>, grid, block, 0, stream, it, prefix, gx, nitems); break;
         default: return hipErrorInvalidValue;
       }
@@ -1139,7 +1156,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
         MVD_BXF4(128)
         MVD_BXF4(144)
         MVD_BXF4(256)
+        MVD_BXF4(192)
         MVD_BXF4(288)
+        MVD_BXF4(384)
         MVD_BXF4(512)
         default: return hipErrorInvalidValue;
       }
@@ -1159,7 +1178,9 @@ hipError_t launch_batch_fast(int kind, int cls, const void* items, const int* pr
         MVD_BXI4(128)
         MVD_BXI4(144)
         MVD_BXI4(256)
+        MVD_BXI4(192)
         MVD_BXI4(288)
+        MVD_BXI4(384)
         MVD_BXI4(512)
         default: return hipErrorInvalidValue;
       }
@@ -1189,7 +1210,9 @@ hipError_t launch_y_fast_window(const ShapePlan& sp, float2* spec, int zlo, int 
     MVD_Y(128)
     MVD_Y(144)
     MVD_Y(256)
+    MVD_Y(192)
     MVD_Y(288)
+    MVD_Y(384)
     MVD_Y(512)
     default: return hipErrorInvalidValue;
   }
@@ -1217,7 +1240,9 @@ hipError_t launch_z_fast(const ShapePlan& sp, float2* spec, const float2* kspec,
     MVD_Z(128)
     MVD_Z(144)
     MVD_Z(256)
+    MVD_Z(192)
     MVD_Z(288)
+    MVD_Z(384)
     MVD_Z(512)
     default: return hipErrorInvalidValue;
   }
@@ -1252,7 +1277,9 @@ hipError_t launch_x_forward_fast(const ShapePlan& sp, const ConvGeom& g, const f
     MVD_XF(128)
     MVD_XF(144)
     MVD_XF(256)
+    MVD_XF(192)
     MVD_XF(288)
+    MVD_XF(384)
     MVD_XF(512)
     default: return hipErrorInvalidValue;
   }
@@ -1285,7 +1312,9 @@ hipError_t launch_x_inverse_fast(const ShapePlan& sp, const ConvGeom& g, const f
     MVD_XI(128)
     MVD_XI(144)
     MVD_XI(256)
+    MVD_XI(192)
     MVD_XI(288)
+    MVD_XI(384)
     MVD_XI(512)
     default: return hipErrorInvalidValue;
   }

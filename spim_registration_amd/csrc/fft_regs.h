@@ -4,7 +4,7 @@
 // Every line is transformed by T = N / E cooperating threads that each keep
 // E complex values in VGPRs.  A length is split into two or three radices
 // (256 = 16 x 16, 128 = 16 x 8, 512 = 8 x 8 x 8, 64 = 8 x 8, 288 = 24 x 12,
-// 144 = 12 x 12); butterflies run in registers and the lines meet in LDS only
+// 144 = 12 x 12, 192 = 24 x 8, 384 = 8 x 8 x 6); butterflies run in registers and the lines meet in LDS only
 // between radix passes -- one LDS exchange for a 256-point line instead of one
 // per radix-2/4 stage.  All sizes are compile-time, so index arithmetic is
 // shifts, masks and multiplications by constants.
@@ -98,6 +98,14 @@ __device__ __forceinline__ void butterfly_pq(float2* v, float dir, const float* 
 }
 
 template <>
+__device__ __forceinline__ void butterfly<6>(float2* v, float dir) {
+  const float r3 = 0.866025403784438646763723f;  // sqrt(3) / 2
+  const float cs[6] = {1.f, 0.5f, -0.5f, -1.f, -0.5f, 0.5f};
+  const float sn[6] = {0.f, r3, r3, 0.f, -r3, -r3};
+  butterfly_pq<2, 3>(v, dir, cs, sn);
+}
+
+template <>
 __device__ __forceinline__ void butterfly<12>(float2* v, float dir) {
   const float r3 = 0.866025403784438646763723f;  // sqrt(3) / 2
   const float cs[12] = {1.f, r3, 0.5f, 0.f, -0.5f, -r3, -1.f, -r3, -0.5f, 0.f, 0.5f, r3};
@@ -135,12 +143,20 @@ struct Spec<144> {
   static constexpr int E = 12, R0 = 12, R1 = 12, R2 = 1, NT = 192;
 };
 template <>
+struct Spec<192> {
+  static constexpr int E = 24, R0 = 24, R1 = 8, R2 = 1, NT = kThreads;
+};
+template <>
 struct Spec<256> {
   static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1, NT = kThreads;
 };
 template <>
 struct Spec<288> {
   static constexpr int E = 24, R0 = 24, R1 = 12, R2 = 1, NT = 192;
+};
+template <>
+struct Spec<384> {
+  static constexpr int E = 24, R0 = 8, R1 = 8, R2 = 6, NT = kThreads;
 };
 template <>
 struct Spec<512> {

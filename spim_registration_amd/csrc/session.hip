@@ -757,8 +757,11 @@ int build_pass_launches(const std::vector<PassDesc>& descs, std::vector<PassLaun
       path = 1;
       build_fast_params(d, tmp, &gx, &gy, &cls);
     }
+    // (the 384-point lines kernel exists with plane tables only: without them
+    // it needs more than 256 VGPRs; those items take the batched z kernel)
     uintptr_t lines = 0;
-    if (path == 1 && d.kind == kPassZConv && d.kspec && kshare[{d.kspec, d.sp}] > 1)
+    if (path == 1 && d.kind == kPassZConv && d.kspec && kshare[{d.kspec, d.sp}] > 1 &&
+        (d.ztab != nullptr || d.sp->q.z != 384))
       lines = reinterpret_cast<uintptr_t>(d.kspec);
     groups[{path, cls, d.kind == kPassXInverse ? d.e.mode : 0, lines}].push_back(&d);
   }
@@ -1160,13 +1163,27 @@ int init_view(mvd_session* s, int v, bool normalise = true) {
 
 // FFT shape of a block window.  MVD_FFT_SMOOTH: smallest 5-smooth size per
 // axis (even along x); MVD_FFT_POW2: smallest power of two (up to 1024, else
-// 5-smooth); MVD_FFT_AUTOTUNE: both candidates per axis are timed on the GPU
-// (one full block convolution each, best of three) and the fastest shape is
-// kept, cached per window shape for the process.
+// 5-smooth); MVD_FFT_AUTOTUNE: per axis the smallest 5-smooth size and the
+// smallest size the register-resident kernels take are timed on the GPU (one
+// full block convolution each, best of three) and the fastest shape is kept,
+// cached per window shape for the process.
 static int pow2_at_least(int n) {
   int p = 1;
   while (p < n) p <<= 1;
   return p;
+}
+
+// Smallest length >= w that the register-resident kernels take (along x: a
+// row of 2M samples with M such a length); the power of two up to 1024 when
+// none is, else the 5-smooth length sm.
+static int fast_at_least(int w, bool x_axis, int sm) {
+  static const int lengths[] = {64, 128, 144, 192, 256, 288, 384, 512};
+  for (int m : lengths) {
+    const int q = x_axis ? 2 * m : m;
+    if (q >= w && fast_size(m)) return q;
+  }
+  const int p = pow2_at_least(w);
+  return p <= 1024 ? p : sm;
 }
 
 std::mutex g_tune_mu;
@@ -1224,12 +1241,13 @@ int time_shape(mvd_session* s, Int3 win, Int3 q, float* ms_out) {
   return 0;
 }
 
-// Fastest FFT shape for a window among the smallest 5-smooth and power-of-two
-// lengths per axis (all 8 combinations), timed on the GPU and cached.
+// Fastest FFT shape for a window among the smallest 5-smooth and the smallest
+// register-kernel lengths per axis (all 8 combinations), timed on the GPU and
+// cached.
 int tune_fft_shape(mvd_session* s, Int3 win, Int3* q, float* ms_out) {
   const Int3 smooth = fft_shape_for(win);
-  auto p2 = [](int w, int sm) { const int p = pow2_at_least(w); return p <= 1024 ? p : sm; };
-  const Int3 pow2 = {p2(win.x, smooth.x), p2(win.y, smooth.y), p2(win.z, smooth.z)};
+  const Int3 reg = {fast_at_least(win.x, true, smooth.x), fast_at_least(win.y, false, smooth.y),
+                    fast_at_least(win.z, false, smooth.z)};
   const auto key = std::make_tuple(win.x, win.y, win.z);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1243,10 +1261,10 @@ int tune_fft_shape(mvd_session* s, Int3 win, Int3* q, float* ms_out) {
   Int3 best = smooth;
   float best_ms = 1e30f;
   for (int c = 0; c < 8; ++c) {
-    const Int3 cand = {(c & 1) ? pow2.x : smooth.x, (c & 2) ? pow2.y : smooth.y, (c & 4) ? pow2.z : smooth.z};
-    if (c && ((c & 1) && pow2.x == smooth.x)) continue;
-    if (c && ((c & 2) && pow2.y == smooth.y)) continue;
-    if (c && ((c & 4) && pow2.z == smooth.z)) continue;
+    const Int3 cand = {(c & 1) ? reg.x : smooth.x, (c & 2) ? reg.y : smooth.y, (c & 4) ? reg.z : smooth.z};
+    if (c && ((c & 1) && reg.x == smooth.x)) continue;
+    if (c && ((c & 2) && reg.y == smooth.y)) continue;
+    if (c && ((c & 4) && reg.z == smooth.z)) continue;
     float ms = 0.f;
     int rc = time_shape(s, win, cand, &ms);
     if (rc) return rc;

@@ -112,15 +112,16 @@ def test_blocked_gradient_matches_oracle(block):
 
 @pytest.mark.parametrize("size,k", [((260, 20, 30), (29, 5, 5)), ((20, 260, 250), (5, 29, 39)),
                                     ((100, 100, 20), (29, 29, 9)), ((260, 270, 24), (29, 19, 5)),
-                                    ((20, 472, 472), (5, 41, 41))])
+                                    ((20, 472, 472), (5, 41, 41)), ((20, 162, 344), (5, 31, 41))])
 def test_gradient_288_point_windows(size, k):
     """Windows of 288 points (image + kernel - 1) along x (a 144-point complex
     transform of the packed row) or along y and z take the 288 = 24 x 12
     register kernels, including the gradient z pass.  With register x and y
     passes (the last two 288 shapes: 128 x 128 and 288 x 288 windows) the
     pass-1 x inverse transforms the residual rows itself (no residual image).
-    The last shape has 512-point y and z windows: the line kernels whose
-    workgroups share 128-byte row lines in pairs, the gradient z pass too."""
+    The shape with 512-point y and z windows takes the line kernels whose
+    workgroups share 128-byte row lines in pairs, the gradient z pass too; the
+    last one 192- and 384-point y and z windows."""
     # start PSFs off by up to 20 % in width, as in a refinement
     ds = S.make_dataset(size, 2, [0, 90], 12, k, 1.2, 2.0, seed=sum(size))
     start = S.perturbed_psfs(k, 1.2, 2.0, [0, 90], seed=sum(size))

@@ -44,6 +44,8 @@ def test_device_query(jna):
     ((64, 64, 64), (13, 11, 9)), ((128, 128, 128), (31, 19, 19)), ((64, 256, 256), (51, 31, 31)),
     ((64, 64, 512), (9, 9, 41)), ((32, 64, 1024), (5, 7, 9)), ((512, 64, 64), (81, 9, 9)),
     ((256, 128, 128), (3, 3, 3)), ((64, 512, 48), (9, 31, 9)),
+    # 192 = 24 x 8 and 384 = 8 x 8 x 6 (x: a 192-point transform of the packed row)
+    ((64, 64, 384), (9, 9, 41)), ((384, 192, 64), (31, 9, 9)),
     # 288- and 144-point lengths (288 = 24 x 12, 144 = 12 x 12) take the
     # register-resident kernels too; 100 (x) the generic ones
     ((288, 144, 144), (31, 19, 19)), ((100, 288, 280), (9, 31, 51)),
@@ -187,7 +189,10 @@ def test_deconvolution_matches_oracle(ptype, lam, block):
                                         ((130, 540, 40), (128, 512, 32)), ((130, 40, 540), (128, 32, 512)),
                                         # 144 rows per plane are not a multiple of the 64-point x
                                         # kernel's 32 rows per workgroup: no per-plane x tasks
-                                        ((150, 160, 80), (128, 144, 64))])
+                                        ((150, 160, 80), (128, 144, 64)),
+                                        # 192- and 384-point register passes (384-point z lines
+                                        # through plane tables)
+                                        ((150, 210, 80), (128, 192, 64)), ((150, 80, 420), (128, 64, 384))])
 def test_block_batch_separate_passes_matches_oracle(size, block):
     """Several blocks whose windows take the separate register x and y passes
     (not the fused 256 x 256 planes): the batched iteration transforms each
