@@ -173,18 +173,21 @@ struct Spec<512> {
 // such access takes an extra cycle.  For 256 and 512 points the rows are L
 // slots wide and offset by i & 16 (256) or i & 24 (512) slots, which puts the
 // groups of a half wave in disjoint banks in every exchange of both transform
-// orders, plus 32 slots per 32 elements so that rows do not overlap.  Other
-// lengths keep rows of L + 1 slots.
+// orders, plus 32 slots per 32 elements so that rows do not overlap.  The
+// 288-point exchange shows no bank conflicts with either row width, and rows of
+// L slots save the LDS that fits a fourth workgroup on a CU.  Other lengths
+// keep rows of L + 1 slots.
 template <int N, int L>
 __device__ __forceinline__ int line_slot(int i, int line) {
   if constexpr (N == 256 && L == 16) return (i << 4) + line + (i & 16) + (i & ~31);
   else if constexpr (N == 512 && L == 8) return (i << 3) + line + (i & 24) + (i & ~31);
+  else if constexpr (N == 288) return i * L + line;
   else return i * (L + 1) + line;
 }
 
 template <int N, int L>
 constexpr int line_slots() {
-  return N * (L + 1);
+  return N == 288 ? N * L : N * (L + 1);
 }
 
 // Butterflies of one pass on the registers (inputs already in pass order).
