@@ -1044,9 +1044,11 @@ int run_conv(mvd_session* s, const ShapePlan& sp, const ConvGeom& g, const float
 // Spectrum of a kernel wrapped into the FFT box of sp, scaled by 1/(qx qy qz).
 int kernel_spectrum(mvd_session* s, const ShapePlan& sp, const std::vector<float>& k, Int3 kd, DeviceBuf* out) {
   MVD_CHECK(out->ensure(sizeof(float2) * sp.spectrum_elems()));
-  // the row padding [hx, pitch) is never written by the transforms; zeros
-  // there keep the (discarded) padding lanes of the z pass finite
-  MVD_CHECK(hipMemsetAsync(out->p, 0, sizeof(float2) * sp.spectrum_elems(), s->stream));
+  // the generic x pass leaves the row padding [hx, pitch) unwritten; zeros
+  // there keep the (discarded) padding lanes of the z pass finite.  The
+  // register x pass stores the padding as zeros with every row.
+  if (!fast_size(sp.q.x / 2))
+    MVD_CHECK(hipMemsetAsync(out->p, 0, sizeof(float2) * sp.spectrum_elems(), s->stream));
   DeviceBuf dk;
   MVD_CHECK(dk.ensure(sizeof(float) * k.size()));
   MVD_CHECK(hipMemcpyAsync(dk.p, k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s->stream));
