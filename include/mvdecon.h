@@ -160,7 +160,9 @@ int mvd_psf_transform_size(const int* psf_size, const double* model, int* new_si
 enum {
   MVD_FFT_SMOOTH = 0,   /* smallest 5-smooth size per axis (default) */
   MVD_FFT_POW2 = 1,     /* smallest power of two per axis (up to 1024) */
-  MVD_FFT_AUTOTUNE = 2  /* time both candidates per axis on the GPU, keep the fastest (cached per shape) */
+  MVD_FFT_AUTOTUNE = 2  /* time the smallest 5-smooth size and the smallest size of the register-resident
+                           kernels (64, 128, 144, 192, 256, 288, 384, 512; twice those along x) per axis
+                           on the GPU, keep the fastest shape (cached per window shape) */
 };
 
 /* new MVDeconInput + image geometry: nviews views of nx*ny*nz voxels on `device`. */
