@@ -253,8 +253,9 @@ __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
   // the workgroup of line (kx, ky) = (0, 0) takes R^(0) from the double sum
   const bool dc = P.rsum && c0 == 0 && y == 0;
   const float dcv = dc ? (float)*gmem(P.rsum) : 0.f;
-  // per-line part of the lag-shift phase (kx = c0 + b, ky = y)
-  const int ty = (int)(((int64_t)y * P.shift.y) % P.q.y);
+  // per-line part of the lag-shift phase (kx = c0 + b, ky = y); the products
+  // of an index and a shift stay below 1024^2, so 32-bit remainders
+  const int ty = (int)(((uint32_t)y * (uint32_t)P.shift.y) % (uint32_t)P.q.y);
   const float2 py = gmem(P.twy)[ty];
   for (int idx = threadIdx.x; idx < qz * B; idx += kThreads) {
     const int z = idx / B, b = idx - z * B;
@@ -265,8 +266,8 @@ __global__ __launch_bounds__(kThreads) void k_z_grad(ZGradParams P) {
       const float2 r = dc && z == 0 && kx == 0 ? make_float2(dcv, 0.f) : buf[z * lp + b];
       if (adj) a = cmul(r, cconj(gmem(P.khat)[o]));
       if (corr) {
-        const int tx = (int)(((int64_t)kx * P.shift.x) % P.q.x);
-        const int tz = (int)(((int64_t)z * P.shift.z) % qz);
+        const int tx = (int)(((uint32_t)kx * (uint32_t)P.shift.x) % (uint32_t)P.q.x);
+        const int tz = (int)(((uint32_t)z * (uint32_t)P.shift.z) % (uint32_t)qz);
         const float2 ph = cmul(cmul(gmem(P.twx)[tx], py), tw[tz]);
         c = cmul(cmul(cconj(r), gmem(P.what)[o]), ph);
         if (P.mode == 0) {

@@ -453,9 +453,10 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
   if (P.rsum && ky == 0 && col == 0 && jt == 0) v[0] = make_float2((float)*gmem(P.rsum), 0.f);
   if (P.corr) {
     // lag-shift phase exp(-2 pi i (kx sx / qx + ky sy / qy + kz sz / qz)): the
-    // (kx, ky) part is one factor per line
-    const int tx = (int)(((int64_t)(valid ? col : 0) * P.shift.x) % P.q.x);
-    const int ty = (int)(((int64_t)ky * P.shift.y) % P.q.y);
+    // (kx, ky) part is one factor per line.  Index times shift stays below
+    // 1024^2: 32-bit remainders (a 64-bit one per element cost c4 5 %)
+    const int tx = (int)(((uint32_t)(valid ? col : 0) * (uint32_t)P.shift.x) % (uint32_t)P.q.x);
+    const int ty = (int)(((uint32_t)ky * (uint32_t)P.shift.y) % (uint32_t)P.q.y);
     const float2 pl = cmul(gmem(P.twx)[tx], gmem(P.twy)[ty]);
     float2 c[E];
 #pragma unroll
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) 
         const int z = elem_index<N, E, RL>(jt, b, t);
         float2 w = make_float2(0.f, 0.f);
         if (valid) w = gmem(P.what)[z * plane + off];
-        const float2 ph = cmul(pl, tw[(int)(((int64_t)z * P.shift.z) % N)]);
+        const float2 ph = cmul(pl, tw[((uint32_t)z * (uint32_t)P.shift.z) % (uint32_t)N]);
         c[b * RL + t] = cmul(cmul(cconj(v[b * RL + t]), w), ph);
       }
     if (P.mode == 0) {
