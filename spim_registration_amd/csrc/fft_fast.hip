@@ -422,8 +422,13 @@ struct ZGradFastParams {
   int adj_add;         // add the adjoint rows to adj instead of storing them
 };
 
+// At 288 points the kernel holds the residual line, the correlation line and
+// their loads in about 220 VGPRs, which allows two waves per SIMD; asking for
+// three (168 VGPRs, about 30 of them spilled to scratch) made the c4 gradient
+// 5 % faster -- the line loads' latency is what bounds it.
 template <int N>
-__global__ __launch_bounds__(Spec<N>::NT) void k_z_fast_grad(ZGradFastParams P) {
+__global__ __launch_bounds__(Spec<N>::NT) __attribute__((amdgpu_waves_per_eu(N == 288 ? 3 : 1)))
+void k_z_fast_grad(ZGradFastParams P) {
   using S = Spec<N>;
   constexpr int E = S::E, T = N / E, NT = S::NT, L = NT / T;
   constexpr int RL = last_radix<N>();
