@@ -424,8 +424,9 @@ struct ZGradFastParams {
 
 // At 288 points the kernel holds the residual line, the correlation line and
 // their loads in about 220 VGPRs, which allows two waves per SIMD; asking for
-// three (168 VGPRs, about 30 of them spilled to scratch) made the c4 gradient
-// 5 % faster -- the line loads' latency is what bounds it.
+// three (168 VGPRs, about 30 of them spilled to scratch) made the gradient of
+// four 256^3 views (288-point window) 5 % faster -- the line loads' latency is
+// what bounds it.
 template <int N>
 __global__ __launch_bounds__(Spec<N>::NT) __attribute__((amdgpu_waves_per_eu(N == 288 ? 3 : 1)))
 void k_z_fast_grad(ZGradFastParams P) {
@@ -459,7 +460,8 @@ void k_z_fast_grad(ZGradFastParams P) {
   if (P.corr) {
     // lag-shift phase exp(-2 pi i (kx sx / qx + ky sy / qy + kz sz / qz)): the
     // (kx, ky) part is one factor per line.  Index times shift stays below
-    // 1024^2: 32-bit remainders (a 64-bit one per element cost c4 5 %)
+    // 1024^2: 32-bit remainders (a 64-bit one per element cost the 288-point
+    // gradient 5 %)
     const int tx = (int)(((uint32_t)(valid ? col : 0) * (uint32_t)P.shift.x) % (uint32_t)P.q.x);
     const int ty = (int)(((uint32_t)ky * (uint32_t)P.shift.y) % (uint32_t)P.q.y);
     const float2 pl = cmul(gmem(P.twx)[tx], gmem(P.twy)[ty]);
