@@ -174,9 +174,9 @@ struct Spec<512> {
 // slots wide and offset by i & 16 (256) or i & 24 (512) slots, which puts the
 // groups of a half wave in disjoint banks in every exchange of both transform
 // orders, plus 32 slots per 32 elements so that rows do not overlap.  The
-// 288-point exchange showed no bank conflicts with rows of L + 1 slots; rows of
-// L slots save the LDS that fits a fourth workgroup on a CU, which measured
-// faster.  Other lengths keep rows of L + 1 slots.
+// 288-point exchange measures no bank conflicts with rows of L + 1 or L slots;
+// rows of L slots save the LDS that fits a fourth workgroup on a CU.  Other
+// lengths keep rows of L + 1 slots.
 template <int N, int L>
 __device__ __forceinline__ int line_slot(int i, int line) {
   if constexpr (N == 256 && L == 16) return (i << 4) + line + (i & 16) + (i & ~31);
